@@ -1,0 +1,36 @@
+// Helpers shared by the extern "C" translation units.
+#pragma once
+
+#include <new>
+#include <string>
+
+#include "bvss.hpp"
+#include "csc.hpp"
+
+struct blest_csc;
+struct blest_bvss;
+
+namespace blest {
+
+extern thread_local std::string tl_last_error;
+blest_status fail(blest_status s, const std::string& msg);
+const CSC& csc_of(const blest_csc* h);
+const BVSS& bvss_of(const blest_bvss* h);
+
+// Runs fn, converting exceptions into a status code (and the thread's last error).
+template <class F>
+blest_status guarded(F&& fn) {
+    try {
+        tl_last_error.clear();
+        fn();
+        return BLEST_OK;
+    } catch (const Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(BLEST_ERR_OUT_OF_HOST_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(BLEST_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+}  // namespace blest

@@ -1,0 +1,336 @@
+// Connected components over the HBM-resident BVSS (gfx950).
+//
+// Reference: BVSSCCKernel (CC/BVSSCCKernel.cuh:33-276 min-label propagation over the
+// BVSS, 1067-1311 host side).  The reference's fixed point is, for an undirected
+// graph, the smallest (reordered) vertex ID of each component; its host compaction
+// (CC/BVSSCCKernel.cuh:1269-1290) is not reproduced, labels are canonicalised to the
+// smallest ORIGINAL vertex ID of the component instead.
+//
+// algo 0 (default): hook + compress.  Every slice (row i, mask m) of rset r is the set
+//   of edges {i, 8r+b : bit b of m}; each undirected edge is hooked once (i < 8r+b).
+//   Roots are linked larger-under-smaller with a compare-and-swap, so every parent is
+//   smaller than its child and the surviving root of a component is its smallest
+//   vertex.  One pass over the slices + one compression pass, independent of diameter.
+// algo 1: min-label propagation, the reference's iteration (frontier = rsets whose
+//   labels dropped), as one persistent cooperative launch.
+#include "dgraph.hpp"
+
+namespace blest {
+namespace dev {
+
+__device__ __forceinline__ uint32_t ld_parent(const uint32_t* p, uint32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving.  Parents only ever decrease and always name an
+// ancestor, so concurrent halving stores keep the forest valid.
+__device__ __forceinline__ uint32_t find_root(uint32_t* p, uint32_t x) {
+    uint32_t par = ld_parent(p, x);
+    while (par != x) {
+        const uint32_t gp = ld_parent(p, par);
+        if (gp == par) return par;
+        __hip_atomic_store(p + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+        par = ld_parent(p, x);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void hook(uint32_t* p, uint32_t a, uint32_t b) {
+    for (;;) {
+        uint32_t ra = find_root(p, a), rb = find_root(p, b);
+        if (ra == rb) return;
+        if (ra > rb) {
+            const uint32_t t = ra;
+            ra = rb;
+            rb = t;
+        }
+        uint32_t expected = rb;
+        if (__hip_atomic_compare_exchange_strong(p + rb, &expected, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+        a = ra;
+        b = expected;  // rb was hooked meanwhile: continue from its new parent
+    }
+}
+
+__global__ void cc_init_kernel(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+__device__ __forceinline__ void hook_slice(uint32_t* p, uint32_t row, uint32_t m, uint32_t colBase) {
+    for (; m; m &= m - 1) {
+        const uint32_t col = colBase + __builtin_ctz(m);
+        if (row < col) hook(p, row, col);
+    }
+}
+
+// Wave per 64 vsets, load-balanced over their quads (as the BFS expansion).
+__global__ void __launch_bounds__(256) cc_hook_kernel(GraphView g, uint32_t* p) {
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
+    for (unsigned base = wave * kWave; base < g.V; base += nwaves * kWave) {
+        const unsigned v = base + lane;
+        uint32_t q0 = 0, qc = 0, r = 0;
+        if (v < g.V) {
+            q0 = g.unitQuad[v];
+            qc = g.unitQuad[v + 1] - q0;
+            r = g.unitRset[v];
+        }
+        const unsigned incl = wave_inclusive_scan(qc);
+        const unsigned excl = incl - qc;
+        const unsigned total = __shfl(incl, kWave - 1, kWave);
+        for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+            const unsigned t = t0 + lane;
+            const unsigned owner = wave_owner(excl, min(t, total - 1));
+            const uint32_t oq0 = __shfl(q0, owner, kWave);
+            const unsigned oex = __shfl(excl, owner, kWave);
+            const uint32_t orr = __shfl(r, owner, kWave);
+            if (t < total) {
+                const uint32_t q = oq0 + (t - oex);
+                const uint4 r4 = g.rows[q];
+                const uint32_t m = g.masks[q];
+                const uint32_t cb = orr * 8;
+                hook_slice(p, r4.x, m & 0xFF, cb);
+                hook_slice(p, r4.y, (m >> 8) & 0xFF, cb);
+                hook_slice(p, r4.z, (m >> 16) & 0xFF, cb);
+                hook_slice(p, r4.w, (m >> 24) & 0xFF, cb);
+            }
+        }
+    }
+}
+
+__global__ void cc_compress_kernel(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t r = find_root(p, i);
+        __hip_atomic_store(p + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* a, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
+// minOrig[root] = smallest original ID in the component.
+__global__ void cc_min_original_kernel(const uint32_t* p, const uint32_t* newToOld, uint32_t* minOrig, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicMin(minOrig + p[i], newToOld[i]);
+}
+
+// out[old] = minOrig[root(new(old))]
+__global__ void cc_relabel_kernel(const uint32_t* p, const uint32_t* oldToNew, const uint32_t* minOrig,
+                                  uint32_t* out, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = minOrig[p[oldToNew[i]]];
+}
+
+// ------------------------------------------------------------ label propagation
+struct LpArgs {
+    uint32_t* labels;
+    unsigned long long* front0;
+    unsigned long long* front1;
+    unsigned long long* marker;
+    uint32_t* list0;
+    uint32_t* list1;
+    Ctrl* ctrl;
+    unsigned long long timeout;
+};
+
+__device__ __forceinline__ void lp_slice(uint32_t* labels, unsigned long long* marker, uint32_t row, uint32_t valid,
+                                         uint32_t colBase) {
+    if (!valid) return;
+    uint32_t best = ld_parent(labels, row);
+    const uint32_t mine = best;
+    for (; valid; valid &= valid - 1) best = min(best, ld_parent(labels, colBase + __builtin_ctz(valid)));
+    if (best < mine) {
+        const uint32_t old = atomicMin(labels + row, best);
+        if (best < old) atomicOr(&marker[row >> 6], 1ull << (row & 63));
+    }
+}
+
+// Wave-cooperative list append (same contract as the BFS one).
+__device__ __forceinline__ void lp_enqueue(const GraphView& g, uint32_t w, unsigned long long nw, uint32_t* list,
+                                           unsigned* counter) {
+    unsigned cnt = 0;
+    if (nw)
+        for (int b = 0; b < 8; ++b)
+            if ((nw >> (8 * b)) & 0xFFull) cnt += g.rsetUnit[w * 8 + b + 1] - g.rsetUnit[w * 8 + b];
+    const unsigned incl = wave_inclusive_scan(cnt);
+    const unsigned total = __shfl(incl, kWave - 1, kWave);
+    if (total == 0) return;
+    unsigned base = 0;
+    if (lane_id() == kWave - 1) base = atomicAdd(counter, total);
+    base = __shfl(base, kWave - 1, kWave);
+    unsigned pos = base + incl - cnt;
+    if (cnt)
+        for (int b = 0; b < 8; ++b)
+            if ((nw >> (8 * b)) & 0xFFull)
+                for (uint32_t u = g.rsetUnit[w * 8 + b]; u < g.rsetUnit[w * 8 + b + 1]; ++u) list[pos++] = u;
+}
+
+__global__ void __launch_bounds__(1024) cc_lp_kernel(GraphView g, LpArgs a) {
+    Ctrl* c = a.ctrl;
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned nthreads = gridDim.x * blockDim.x;
+    const unsigned wave = tid / kWave, nwaves = nthreads / kWave;
+    unsigned gen = 0;
+    if (threadIdx.x == 0) gen = ld_acquire(&c->barGen);
+    // All vertices start active; every vset is in the first list.
+    for (uint32_t i = tid; i < g.n; i += nthreads) a.labels[i] = i;
+    for (uint32_t w = tid; w < g.W; w += nthreads) {
+        const uint32_t lo = w * 64;
+        const uint32_t cnt = min(64u, g.n - lo);
+        a.front1[w] = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
+        a.marker[w] = 0ull;
+    }
+    for (uint32_t v = tid; v < g.V; v += nthreads) a.list1[v] = v;
+    if (tid == 0) {
+        for (int s = 0; s < 3; ++s) c->listCount[s] = 0;
+        c->listCount[1] = g.V;
+    }
+    if (!grid_sync(c, gen, a.timeout)) return;
+    unsigned it = 1;
+    for (;; ++it) {
+        const unsigned sIn = it % 3, sOut = (it + 1) % 3, sRst = (it + 2) % 3;
+        const unsigned cnt = ld_relaxed(&c->listCount[sIn]);
+        if (cnt == 0) break;
+        const unsigned long long* fin = (it & 1) ? a.front1 : a.front0;
+        unsigned long long* fout = (it & 1) ? a.front0 : a.front1;
+        uint32_t* lin = (it & 1) ? a.list1 : a.list0;
+        uint32_t* lout = (it & 1) ? a.list0 : a.list1;
+        if (tid == 0) __hip_atomic_store(&c->listCount[sRst], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
+            const unsigned e = base + lane;
+            uint32_t q0 = 0, qc = 0, fb = 0, r = 0;
+            if (e < cnt) {
+                const uint32_t v = lin[e];
+                q0 = g.unitQuad[v];
+                qc = g.unitQuad[v + 1] - q0;
+                r = g.unitRset[v];
+                fb = reinterpret_cast<const uint8_t*>(fin)[r];
+                if (!fb) qc = 0;
+            }
+            const unsigned incl = wave_inclusive_scan(qc);
+            const unsigned excl = incl - qc;
+            const unsigned total = __shfl(incl, kWave - 1, kWave);
+            for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+                const unsigned t = t0 + lane;
+                const unsigned owner = wave_owner(excl, min(t, total - 1));
+                const uint32_t oq0 = __shfl(q0, owner, kWave);
+                const unsigned oex = __shfl(excl, owner, kWave);
+                const uint32_t of = __shfl(fb, owner, kWave);
+                const uint32_t orr = __shfl(r, owner, kWave);
+                if (t < total) {
+                    const uint32_t q = oq0 + (t - oex);
+                    const uint4 r4 = g.rows[q];
+                    const uint32_t m = g.masks[q];
+                    const uint32_t cb = orr * 8;
+                    lp_slice(a.labels, a.marker, r4.x, m & of & 0xFF, cb);
+                    lp_slice(a.labels, a.marker, r4.y, (m >> 8) & of & 0xFF, cb);
+                    lp_slice(a.labels, a.marker, r4.z, (m >> 16) & of & 0xFF, cb);
+                    lp_slice(a.labels, a.marker, r4.w, (m >> 24) & of & 0xFF, cb);
+                }
+            }
+        }
+        if (!grid_sync(c, gen, a.timeout)) return;
+        for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
+            const uint32_t w = w0 + lane;
+            unsigned long long mk = 0;
+            if (w < g.W) {
+                mk = a.marker[w];
+                if (mk) a.marker[w] = 0ull;
+                fout[w] = mk;
+            }
+            lp_enqueue(g, w, mk, lout, &c->listCount[sOut]);
+        }
+        if (!grid_sync(c, gen, a.timeout)) return;
+    }
+    if (tid == 0) c->iterations = it - 1;
+}
+
+}  // namespace dev
+
+// ------------------------------------------------------------------------ host
+static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st) {
+    DeviceGraph& g = *q.g;
+    if (!g.symmetric) throw Error(BLEST_ERR_UNSUPPORTED, "connected components need a symmetric graph");
+    if (!q.labels) {
+        q.labels = q.alloc<uint32_t>(g.n);
+        q.labelsAux = q.alloc<uint32_t>(g.n);
+    }
+    GraphView gv = g.view();
+    const dim3 grid(uint32_t(g.cus) * 8), block(256);
+    unsigned iterations = 1;
+    BLEST_HIP(hipEventRecord(q.ev0, q.stream));
+    if (algo == 0) {
+        hipLaunchKernelGGL(dev::cc_init_kernel, grid, block, 0, q.stream, q.labels, g.n);
+        hipLaunchKernelGGL(dev::cc_hook_kernel, grid, block, 0, q.stream, gv, q.labels);
+        hipLaunchKernelGGL(dev::cc_compress_kernel, grid, block, 0, q.stream, q.labels, g.n);
+        BLEST_HIP(hipGetLastError());
+    } else if (algo == 1) {
+        dev::LpArgs a;
+        a.labels = q.labels;
+        a.front0 = q.front0;
+        a.front1 = q.front1;
+        a.marker = q.next;
+        a.list0 = q.list0;
+        a.list1 = q.list1;
+        a.ctrl = q.ctrl;
+        a.timeout = timeout_ticks(g);
+        void* args[] = {&gv, &a};
+        launch_cooperative(reinterpret_cast<const void*>(&dev::cc_lp_kernel), g, args, q.stream);
+    } else {
+        throw Error(BLEST_ERR_INVALID_ARGUMENT, "unknown CC algorithm");
+    }
+    BLEST_HIP(hipEventRecord(q.ev1, q.stream));
+    BLEST_HIP(hipStreamSynchronize(q.stream));
+    if (algo == 1) {
+        check_ctrl_error(q);
+        dev::Ctrl c;
+        BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
+        iterations = c.iterations;
+    }
+    float ms = 0;
+    BLEST_HIP(hipEventElapsedTime(&ms, q.ev0, q.ev1));
+    const uint32_t* result = q.labels;
+    if (g.hasPerm) {
+        hipLaunchKernelGGL(dev::fill_u32_kernel, grid, block, 0, q.stream, q.labelsAux, 0xFFFFFFFFu, g.n);
+        hipLaunchKernelGGL(dev::cc_min_original_kernel, grid, block, 0, q.stream, q.labels, g.permNewToOld,
+                           q.labelsAux, g.n);
+        hipLaunchKernelGGL(dev::cc_relabel_kernel, grid, block, 0, q.stream, q.labels, g.permOldToNew, q.labelsAux,
+                           reinterpret_cast<uint32_t*>(q.levels), g.n);
+        BLEST_HIP(hipGetLastError());
+        result = reinterpret_cast<const uint32_t*>(q.levels);
+    }
+    std::vector<uint32_t> tmp;
+    uint32_t* dst = hostLabels;
+    if (!dst) {
+        tmp.resize(g.n);
+        dst = tmp.data();
+    }
+    BLEST_HIP(hipMemcpyAsync(dst, result, 4 * size_t(g.n), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipStreamSynchronize(q.stream));
+    if (st) {
+        uint32_t comps = 0;
+        for (uint32_t v = 0; v < g.n; ++v) comps += dst[v] == v;
+        st->kernel_ms = ms;
+        st->no_components = comps;
+        st->iterations = iterations;
+    }
+}
+
+}  // namespace blest
+
+using namespace blest;
+
+extern "C" blest_status blest_cc(blest_query* qh, int algo, uint32_t* labels, blest_cc_stats* st) {
+    return guarded([&] {
+        if (!qh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "query is null");
+        Query& q = qh->q;
+        std::lock_guard<std::mutex> lk(q.mu);
+        BLEST_HIP(hipSetDevice(q.g->device));
+        cc_run(q, algo, labels, st);
+    });
+}

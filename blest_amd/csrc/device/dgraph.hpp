@@ -1,0 +1,135 @@
+// Host-side handles of the HBM-resident graph and of per-query device state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../capi_internal.hpp"
+#include "dev_common.hpp"
+
+namespace blest {
+
+#define BLEST_HIP(call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::blest::Error(e_ == hipErrorOutOfMemory ? BLEST_ERR_OUT_OF_DEVICE_MEMORY    \
+                                                           : BLEST_ERR_DEVICE,                 \
+                                 std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// Read-only view of the device graph passed by value to kernels.
+//
+// HBM layout: the BVSS of the reference (sliceSetPtrs, virtualToReal, realPtrs, rowIds,
+// masks) is uploaded with slice offsets converted to quad offsets: every vset holds a
+// multiple of 4 slices (BVSS.cuh:379-415), so its slices are whole "quads" of 4 rows
+// (one 16-byte uint4) and one 32-bit mask word (4 x 8-bit slice masks).  A wave64
+// streams 64 quads = 256 slices per coalesced step (1 KiB of rows + 256 B of masks).
+// Bitmaps (visited, frontier) are 64-bit words: word w holds vertices [64w, 64w+64),
+// byte b of the bitmap is the frontier mask of real slice set b.
+struct GraphView {
+    uint32_t n, R, V, W;          // vertices, rsets, vsets, 64-bit bitmap words
+    const uint32_t* unitQuad;     // V + 1: first quad of each vset
+    const uint32_t* unitRset;     // V: owning rset (virtualToReal)
+    const uint32_t* rsetUnit;     // R + 1: first vset of each rset (realPtrs)
+    const uint4* rows;            // Q quads of row IDs (pad rows 0xFFFFFFFF)
+    const uint32_t* masks;        // Q mask words
+    const uint32_t* csrPtr;       // n + 1, bottom-up in-neighbour lists (nullable)
+    const uint32_t* csrIdx;       // nnz
+    unsigned long long nnz;       // directed edges
+};
+
+struct DeviceGraph {
+    int device = 0;
+    int cus = 0;
+    uint32_t blockSize = 512, gridBlocks = 0;
+    double ticksPerMs = 100000.0;  // wall_clock64 frequency
+    blest_device_opts opts{};
+    uint32_t n = 0, R = 0, V = 0, W = 0;
+    uint64_t S = 0, Q = 0, nnz = 0;
+    bool symmetric = false, hasCsr = false, hasPerm = false;
+    uint32_t *unitQuad = nullptr, *unitRset = nullptr, *rsetUnit = nullptr, *masks = nullptr;
+    uint4* rows = nullptr;
+    uint32_t *csrPtr = nullptr, *csrIdx = nullptr;
+    uint32_t *permNewToOld = nullptr, *permOldToNew = nullptr;  // device copies
+    std::vector<uint32_t> hostOldToNew, hostNewToOld;
+    std::vector<uint32_t> hostDegree;  // per (reordered) vertex, for GTEPS accounting
+    uint64_t deviceBytes = 0;
+    std::vector<void*> allocations;
+
+    GraphView view() const {
+        GraphView g;
+        g.n = n;
+        g.R = R;
+        g.V = V;
+        g.W = W;
+        g.unitQuad = unitQuad;
+        g.unitRset = unitRset;
+        g.rsetUnit = rsetUnit;
+        g.rows = rows;
+        g.masks = masks;
+        g.csrPtr = csrPtr;
+        g.csrIdx = csrIdx;
+        g.nnz = nnz;
+        return g;
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        BLEST_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        allocations.push_back(p);
+        deviceBytes += count * sizeof(T);
+        return static_cast<T*>(p);
+    }
+    ~DeviceGraph();
+};
+
+// Mutable per-query device state.
+struct Query {
+    DeviceGraph* g = nullptr;
+    uint32_t maxSources = 1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    dev::Ctrl* ctrl = nullptr;
+    int32_t* levels = nullptr;  // n * maxSources
+    unsigned long long *visited = nullptr, *front0 = nullptr, *front1 = nullptr, *next = nullptr;
+    uint32_t *list0 = nullptr, *list1 = nullptr;
+    int32_t* levelsOut = nullptr;  // original-order copy (when the graph carries a permutation)
+    uint32_t* labels = nullptr;    // CC
+    uint32_t* labelsAux = nullptr;
+    // multi-source state, allocated on first use: words per vertex x n
+    uint32_t mbfsWords = 0;
+    unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
+    std::vector<void*> allocations;
+    std::mutex mu;
+    uint32_t lastSource = 0;
+    bool pending = false;
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        BLEST_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        allocations.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~Query();
+};
+
+// Launch helpers (graph.hip).
+void launch_cooperative(const void* kernel, const DeviceGraph& g, void** args, hipStream_t s);
+void check_ctrl_error(Query& q);
+unsigned long long timeout_ticks(const DeviceGraph& g);
+void bfs_launch(Query& q, uint32_t srcReordered);
+void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);
+
+}  // namespace blest
+
+struct blest_device_graph {
+    blest::DeviceGraph g;
+};
+struct blest_query {
+    blest::Query q;
+};

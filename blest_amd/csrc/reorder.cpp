@@ -1,0 +1,421 @@
+// Vertex reorderings (host).  Results of BFS and CC do not depend on the ordering;
+// the ordering only shapes the BVSS slices (how many rows share a slice) and with
+// that the bytes the traversal streams.
+#include "reorder.hpp"
+
+#include <cmath>
+#include <numeric>
+
+namespace blest {
+
+std::vector<uint32_t> order_natural(const CSC& g) {
+    std::vector<uint32_t> inv(g.n);
+    std::iota(inv.begin(), inv.end(), 0u);
+    return inv;
+}
+
+// CSC::random (CSC.cuh:535-554), seeded instead of std::random_device.
+std::vector<uint32_t> order_random(const CSC& g, uint64_t seed) {
+    std::vector<uint32_t> perm = seeded_permutation(g.n, seed, 11);
+    std::vector<uint32_t> inv(g.n);
+    for (uint32_t i = 0; i < g.n; ++i) inv[perm[i]] = i;
+    return inv;
+}
+
+// CSC::degreeSort (CSC.cuh:968-992): by degree, ties by vertex ID.
+std::vector<uint32_t> order_degree(const CSC& g, bool ascending) {
+    std::vector<uint32_t> ids(g.n);
+    std::iota(ids.begin(), ids.end(), 0u);
+    std::sort(ids.begin(), ids.end(), [&](uint32_t x, uint32_t y) {
+        uint32_t dx = g.degree(x), dy = g.degree(y);
+        if (dx != dy) return ascending ? dx < dy : dx > dy;
+        return x < y;
+    });
+    std::vector<uint32_t> inv(g.n);
+    for (uint32_t k = 0; k < g.n; ++k) inv[ids[k]] = k;
+    return inv;
+}
+
+// ------------------------------------------------------------------------- RCM
+namespace {
+
+// findPseudoPeripheralNode (CSC.cuh:994-1057): repeated BFS from a random vertex of
+// the previous BFS's last level until the depth has not changed for 10 rounds.  The
+// reference draws the restart with glibc rand(); here it is a seeded counter draw.
+uint32_t pseudo_peripheral(const CSC& s, uint32_t start, std::vector<uint32_t>& mark,
+                           uint32_t& stamp, std::vector<uint32_t>& q, uint64_t key,
+                           uint64_t& draw) {
+    uint32_t maxLevel = 0, trial = 0;
+    while (trial < 10) {
+        ++stamp;
+        uint32_t qs = 0, qe = 0;
+        q[qe++] = start;
+        mark[start] = stamp;
+        uint32_t levelStart = 0, levelEnd = 1, levels = 1;
+        while (qs < qe) {
+            uint32_t u = q[qs++];
+            for (uint32_t p = s.colPtrs[u]; p < s.colPtrs[u + 1]; ++p) {
+                uint32_t v = s.rows[p];
+                if (mark[v] != stamp) {
+                    mark[v] = stamp;
+                    q[qe++] = v;
+                }
+            }
+            if (qs == levelEnd && qs < qe) {
+                levelStart = qs;
+                levelEnd = qe;
+                ++levels;
+            }
+        }
+        trial = (levels == maxLevel) ? trial + 1 : 0;
+        maxLevel = levels;
+        start = q[levelStart + uint32_t(crand(key, draw++) % (levelEnd - levelStart))];
+    }
+    return start;
+}
+
+}  // namespace
+
+// CSC::rcm (CSC.cuh:1059-1142): per component, BFS from a pseudo-peripheral vertex,
+// neighbours visited in ascending degree (stable here; std::sort in the reference),
+// numbered in BFS order (the reference does not reverse it despite the name).
+std::vector<uint32_t> order_rcm(const CSC& g, uint64_t seed) {
+    CSC sym;
+    const CSC* s = &g;
+    if (!csc_is_symmetric(g)) {
+        sym = csc_symmetrize(g);
+        s = &sym;
+    }
+    const uint32_t n = g.n;
+    std::vector<uint32_t> inv(n), q(std::max<uint32_t>(n, 1)), mark(n, 0), pq(std::max<uint32_t>(n, 1)),
+        nbrs;
+    std::vector<uint8_t> visited(n, 0);
+    uint32_t stamp = 0, permuted = 0;
+    uint64_t key = stream_key(seed, 12), draw = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (visited[i]) continue;
+        uint32_t start = pseudo_peripheral(*s, i, mark, stamp, pq, key, draw);
+        uint32_t qs = 0, qe = 0;
+        q[qe++] = start;
+        visited[start] = 1;
+        while (qs < qe) {
+            uint32_t u = q[qs++];
+            inv[u] = permuted++;
+            nbrs.clear();
+            for (uint32_t p = s->colPtrs[u]; p < s->colPtrs[u + 1]; ++p) {
+                uint32_t v = s->rows[p];
+                if (!visited[v]) {
+                    visited[v] = 1;
+                    nbrs.push_back(v);
+                }
+            }
+            std::stable_sort(nbrs.begin(), nbrs.end(),
+                             [&](uint32_t x, uint32_t y) { return s->degree(x) < s->degree(y); });
+            for (uint32_t v : nbrs) q[qe++] = v;
+        }
+    }
+    return inv;
+}
+
+// -------------------------------------------------------------- Jaccard window
+// jackardWithWindow (CSC.cuh:717-920).  Columns are grouped into windows of `window`
+// consecutive IDs that are ordered independently (and in parallel); a column never
+// leaves its window.  Within a window, each run of sliceSize new positions (one
+// future real slice set) starts from the unplaced column of largest degree (ties:
+// largest ID) and is filled greedily with the candidate of largest Jaccard similarity
+// inter / (union + deg - inter) to the union of rows already in the run (first best
+// wins ties); candidates are unplaced window columns sharing a row with the run.
+// When no candidate is left the next singleton is taken.
+std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uint32_t window,
+                                           unsigned threads) {
+    if (sliceSize == 0 || window == 0 || window % sliceSize != 0)
+        throw Error(BLEST_ERR_INVALID_ARGUMENT, "Jaccard window must be a positive multiple of the slice size");
+    threads = hw_threads(threads);
+    const uint32_t n = g.n;
+    CSC tr;
+    const CSC* t = &g;  // rows -> columns index: the transpose (g itself when symmetric)
+    if (!csc_is_symmetric(g, threads)) {
+        tr = csc_transpose(g, threads);
+        t = &tr;
+    }
+    std::vector<uint32_t> inv(n);
+    const uint32_t noWindows = (n + window - 1) / window;
+    struct Scratch {
+        std::vector<uint8_t> placed;     // per window column
+        std::vector<uint32_t> inter;     // per window column
+        std::vector<uint32_t> queue;     // candidate columns
+        std::vector<uint32_t> rowMark;   // per row: epoch of the run that holds it
+        std::vector<uint32_t> rowStart;  // per row: first transpose position >= windowStart
+        std::vector<uint32_t> rowWin;    // per row: window for which rowStart is valid (+1)
+        std::vector<uint32_t> bySize;    // window columns by (degree desc, id desc)
+        uint32_t epoch = 0;
+    };
+    std::vector<Scratch> scratch(threads);
+    parallel_dynamic(noWindows, threads, 1, [&](uint64_t w, unsigned tid) {
+        Scratch& S = scratch[tid];
+        const uint32_t ws = uint32_t(w) * window, we = std::min<uint64_t>(uint64_t(ws) + window, n);
+        const uint32_t len = we - ws;
+        if (S.rowMark.empty()) {
+            S.rowMark.assign(n, 0);
+            S.rowStart.assign(n, 0);
+            S.rowWin.assign(n, 0);
+        }
+        S.placed.assign(len, 0);
+        S.inter.assign(len, 0);
+        S.queue.clear();
+        S.bySize.resize(len);
+        std::iota(S.bySize.begin(), S.bySize.end(), ws);
+        std::sort(S.bySize.begin(), S.bySize.end(), [&](uint32_t x, uint32_t y) {
+            uint32_t dx = g.degree(x), dy = g.degree(y);
+            return dx != dy ? dx > dy : x > y;
+        });
+        uint32_t cursor = 0;
+        auto nextSingleton = [&](uint32_t& col) -> bool {
+            while (cursor < len && S.placed[S.bySize[cursor] - ws]) ++cursor;
+            if (cursor == len) return false;
+            col = S.bySize[cursor];
+            return true;
+        };
+        uint32_t unionSize = 0;
+        // Adds every row of column `col` not yet in the run's union, and for each such
+        // row counts it into the intersection of every unplaced window column holding it.
+        auto absorb = [&](uint32_t col) {
+            for (uint32_t p = g.colPtrs[col]; p < g.colPtrs[col + 1]; ++p) {
+                uint32_t r = g.rows[p];
+                if (S.rowMark[r] == S.epoch) continue;
+                S.rowMark[r] = S.epoch;
+                ++unionSize;
+                if (S.rowWin[r] != w + 1) {
+                    const uint32_t* b = t->rows.data() + t->colPtrs[r];
+                    const uint32_t* e = t->rows.data() + t->colPtrs[r + 1];
+                    S.rowStart[r] = uint32_t(std::lower_bound(b, e, ws) - t->rows.data());
+                    S.rowWin[r] = uint32_t(w) + 1;
+                }
+                for (uint32_t q = S.rowStart[r]; q < t->colPtrs[r + 1]; ++q) {
+                    uint32_t c = t->rows[q];
+                    if (c >= we) break;
+                    if (S.placed[c - ws]) continue;
+                    if (S.inter[c - ws]++ == 0) S.queue.push_back(c);
+                }
+            }
+        };
+        const uint32_t noSets = (len + sliceSize - 1) / sliceSize;
+        for (uint32_t set = 0; set < noSets; ++set) {
+            for (uint32_t c : S.queue) S.inter[c - ws] = 0;
+            S.queue.clear();
+            ++S.epoch;
+            unionSize = 0;
+            const uint32_t ss = ws + set * sliceSize, se = std::min(ss + sliceSize, we);
+            uint32_t col;
+            nextSingleton(col);
+            S.placed[col - ws] = 1;
+            absorb(col);
+            inv[col] = ss;
+            for (uint32_t nc = ss + 1; nc < se; ++nc) {
+                uint32_t best;
+                if (S.queue.empty()) {
+                    if (!nextSingleton(best)) break;
+                } else {
+                    uint32_t bi = 0;
+                    double bv = -1.0;
+                    for (uint32_t qi = 0; qi < S.queue.size(); ++qi) {
+                        uint32_t c = S.queue[qi];
+                        uint32_t inter = S.inter[c - ws];
+                        uint32_t uni = unionSize + g.degree(c) - inter;
+                        double v = uni == 0 ? 1.0 : double(inter) / double(uni);
+                        if (v > bv) {
+                            bv = v;
+                            bi = qi;
+                        }
+                    }
+                    best = S.queue[bi];
+                    S.inter[best - ws] = 0;
+                    S.queue[bi] = S.queue.back();
+                    S.queue.pop_back();
+                }
+                S.placed[best - ws] = 1;
+                absorb(best);
+                inv[best] = nc;
+            }
+        }
+    });
+    return inv;
+}
+
+// ---------------------------------------------------------------------- Gorder
+namespace {
+
+// Exact max-priority over integer keys with +-1 updates: one doubly linked list per
+// key and a pointer to the largest non-empty key.
+class BucketQueue {
+public:
+    explicit BucketQueue(uint32_t n) : key_(n, 0), prev_(n), next_(n), in_(n, 1), head_(1, -1) {
+        for (uint32_t v = 0; v < n; ++v) {
+            prev_[v] = int64_t(v) - 1;
+            next_[v] = (v + 1 < n) ? int64_t(v) + 1 : -1;
+        }
+        if (n) head_[0] = 0;
+        top_ = 0;
+        size_ = n;
+    }
+    bool contains(uint32_t v) const { return in_[v] != 0; }
+    uint64_t size() const { return size_; }
+    void remove(uint32_t v) {
+        unlink(v);
+        in_[v] = 0;
+        --size_;
+    }
+    void inc(uint32_t v) {
+        if (!in_[v]) return;
+        unlink(v);
+        ++key_[v];
+        link(v);
+        if (key_[v] > top_) top_ = key_[v];
+    }
+    void dec(uint32_t v) {
+        if (!in_[v]) return;
+        unlink(v);
+        --key_[v];
+        link(v);
+    }
+    // Largest key; among equal keys the most recently linked vertex.
+    int64_t pop_max() {
+        while (top_ > 0 && head_[top_] < 0) --top_;
+        int64_t v = head_[top_];
+        if (v >= 0) remove(uint32_t(v));
+        return v;
+    }
+
+private:
+    void unlink(uint32_t v) {
+        uint64_t k = key_[v];
+        if (prev_[v] >= 0) next_[prev_[v]] = next_[v];
+        else head_[k] = next_[v];
+        if (next_[v] >= 0) prev_[next_[v]] = prev_[v];
+    }
+    void link(uint32_t v) {
+        uint64_t k = key_[v];
+        if (k >= head_.size()) head_.resize(std::max<uint64_t>(k + 1, head_.size() * 2), -1);
+        prev_[v] = -1;
+        next_[v] = head_[k];
+        if (head_[k] >= 0) prev_[head_[k]] = v;
+        head_[k] = v;
+    }
+    std::vector<uint64_t> key_;
+    std::vector<int64_t> prev_, next_;
+    std::vector<uint8_t> in_;
+    std::vector<int64_t> head_;
+    uint64_t top_, size_;
+};
+
+}  // namespace
+
+// Gorder greedy (Wei, Yu, Lu, Lin, "Speedup Graph Processing by Graph Ordering",
+// SIGMOD 2016; vendored by the reference at Gorder/Graph.cpp:388-617).  Places the
+// vertex maximising the Gorder score S(v, w) = |in(v) & in(w)| + [v->w] + [w->v]
+// summed over the last `window` placed vertices w.  As in the reference, in-neighbours
+// whose out-degree exceeds sqrt(n) contribute no sibling score, the first vertex is
+// the one of largest in-degree, and vertices without edges go last.  Priorities are
+// kept exact (the reference's UnitHeap applies decrements lazily).  Unlike the
+// reference (CSC.cuh:936-943) the order is returned in the caller's vertex IDs.
+std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
+    const uint32_t n = g.n;
+    std::vector<uint32_t> inv(n);
+    if (n == 0) return inv;
+    if (window == 0) window = 1;
+    // out(j) = column j's rows (edges j -> i); in(i) = transpose.
+    CSC tr;
+    const CSC* in = &g;
+    if (!csc_is_symmetric(g)) {
+        tr = csc_transpose(g);
+        in = &tr;
+    }
+    auto outDeg = [&](uint32_t v) { return g.colPtrs[v + 1] - g.colPtrs[v]; };
+    auto inDeg = [&](uint32_t v) { return in->colPtrs[v + 1] - in->colPtrs[v]; };
+    const uint64_t huge = uint64_t(std::sqrt(double(n)));
+    BucketQueue pq(n);
+    std::vector<uint32_t> order, zero;
+    order.reserve(n);
+    for (uint32_t v = 0; v < n; ++v)
+        if (outDeg(v) + inDeg(v) == 0) {
+            zero.push_back(v);
+            pq.remove(v);
+        }
+    // Score change of every vertex related to v when v enters (+1) or leaves (-1) the window.
+    auto touch = [&](uint32_t v, bool add) {
+        auto upd = [&](uint32_t x) { add ? pq.inc(x) : pq.dec(x); };
+        if (outDeg(v) <= huge)
+            for (uint32_t p = g.colPtrs[v]; p < g.colPtrs[v + 1]; ++p) upd(g.rows[p]);
+        for (uint32_t p = in->colPtrs[v]; p < in->colPtrs[v + 1]; ++p) {
+            uint32_t u = in->rows[p];
+            if (outDeg(u) > huge) continue;
+            upd(u);
+            if (outDeg(u) > 1)
+                for (uint32_t q = g.colPtrs[u]; q < g.colPtrs[u + 1]; ++q) upd(g.rows[q]);
+        }
+    };
+    if (pq.size() > 0) {
+        uint32_t first = 0;
+        int64_t best = -1;
+        for (uint32_t v = 0; v < n; ++v)
+            if (pq.contains(v) && int64_t(inDeg(v)) > best) {
+                best = inDeg(v);
+                first = v;
+            }
+        pq.remove(first);
+        order.push_back(first);
+        touch(first, true);
+        // v is chosen against the last `window` placed vertices; then the vertex that
+        // falls out of the window is retired and v's contributions are added.
+        while (pq.size() > 0) {
+            int64_t v = pq.pop_max();
+            order.push_back(uint32_t(v));
+            if (order.size() > window) touch(order[order.size() - 1 - window], false);
+            touch(uint32_t(v), true);
+        }
+    }
+    order.insert(order.end(), zero.begin(), zero.end());
+    for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
+    return inv;
+}
+
+std::vector<uint32_t> chain_permutations(const std::vector<uint32_t>& first,
+                                         const std::vector<uint32_t>& second) {
+    std::vector<uint32_t> out(first.size());
+    for (size_t i = 0; i < first.size(); ++i) out[i] = second[first[i]];
+    return out;
+}
+
+std::vector<uint32_t> reorder(CSC& g, const blest_reorder_opts& o) {
+    const uint32_t slice = o.slice_size ? o.slice_size : 8;
+    const unsigned threads = hw_threads(o.threads);
+    blest_reorder_kind kind = o.kind;
+    if (kind == BLEST_REORDER_AUTO) {  // CSC::reorder (CSC.cuh:358-386)
+        if (g.isSocial)
+            kind = o.jaccard ? (o.gorder ? BLEST_REORDER_GORDER_JACCARD : BLEST_REORDER_JACCARD)
+                             : BLEST_REORDER_NATURAL;
+        else
+            kind = BLEST_REORDER_RCM;
+    }
+    std::vector<uint32_t> inv;
+    switch (kind) {
+        case BLEST_REORDER_NATURAL: inv = order_natural(g); break;
+        case BLEST_REORDER_RANDOM: inv = order_random(g, o.seed); break;
+        case BLEST_REORDER_DEGREE: inv = order_degree(g, true); break;
+        case BLEST_REORDER_RCM: inv = order_rcm(g, o.seed); break;
+        case BLEST_REORDER_JACCARD: inv = order_jaccard_window(g, slice, o.window, threads); break;
+        case BLEST_REORDER_GORDER: inv = order_gorder(g, o.window); break;
+        case BLEST_REORDER_GORDER_JACCARD: {  // CSC::gorderWithJackard (CSC.cuh:932-948)
+            std::vector<uint32_t> first = order_gorder(g, o.window);
+            csc_apply_permutation(g, first.data(), threads);
+            std::vector<uint32_t> second = order_jaccard_window(g, slice, o.window, threads);
+            csc_apply_permutation(g, second.data(), threads);
+            return chain_permutations(first, second);
+        }
+        default: throw Error(BLEST_ERR_INVALID_ARGUMENT, "unknown reorder kind");
+    }
+    csc_apply_permutation(g, inv.data(), threads);
+    return inv;
+}
+
+}  // namespace blest
