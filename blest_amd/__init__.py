@@ -306,6 +306,7 @@ class BFSResult:
     td_levels: int = 0
     bu_levels: int = 0
     edges_in_component: int = 0
+    bottom_up_mask: int = 0
 
 
 @dataclass
@@ -353,7 +354,7 @@ class BVSSBFSKernel(BFSKernel):
         check(_lib.load().blest_bfs(self.q._h, int(source_vertex), levels.ctypes.data_as(_lib.I32P),
                                     C.byref(st)))
         return BFSResult(st.kernel_ms / 1e3, int(source_vertex), levels, st.total_levels, st.no_visited,
-                         st.td_levels, st.bu_levels, st.edges_traversed_component)
+                         st.td_levels, st.bu_levels, st.edges_traversed_component, st.bottom_up_mask)
 
 
 class CCKernel:

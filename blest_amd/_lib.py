@@ -54,7 +54,7 @@ class DeviceGraphInfo(C.Structure):
 class BfsStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_levels", C.c_uint32), ("no_visited", C.c_uint32),
                 ("td_levels", C.c_uint32), ("bu_levels", C.c_uint32),
-                ("edges_traversed_component", C.c_uint64)]
+                ("edges_traversed_component", C.c_uint64), ("bottom_up_mask", C.c_uint64)]
 
 
 class MbfsOpts(C.Structure):

@@ -3,28 +3,32 @@
 //
 // Reference path: BVSSBFSKernel::hostCode and its persistent cooperative kernels
 // (BFS/BVSSBFSKernel.cuh:33-284 top-down/bottom-up switching, 531-691 top-down only,
-// 1283-1608 host side).  The whole traversal is one cooperative launch; levels are
-// separated by a grid barrier whose waits are time-bounded.
+// 1283-1608 host side).  On MI355X a grid-wide barrier costs 7-33 us while a kernel
+// boundary on one stream costs ~2.3 us (1.6 us replayed from a graph), so the level
+// loop is a stream of small kernels instead of one cooperative launch: two kernels
+// per level, every decision taken on the device, the host only polling a pinned
+// `done` flag once per chunk of levels.
 //
-// Per level (L = 1, 2, ...; frontier F = vertices at distance L-1):
-//   top-down  A: each wave takes 64 frontier vsets, scans their quad counts across the
-//                wave and streams the union of their quads 64 at a time (every lane
-//                busy, loads coalesced); a slice hits when (mask & frontier byte) != 0;
-//                rows not yet visited are OR-ed into `next` (64-bit atomics).
-//             B: thread per bitmap word: new = next[w]; visited |= new; frontier' = new;
-//                levels of new vertices = L; vsets of rsets with new bits are appended
-//                to the next frontier list (one atomic per wave).
-//   bottom-up: wave per 64-vertex word, lane per vertex: an unvisited vertex scans its
-//              in-neighbours until one is in the frontier; the wave's ballot is the new
-//              frontier word (no atomics, no list; the list is rebuilt from the bitmap
-//              only when a top-down level follows).
-// The direction switch uses edge counts (top-down -> bottom-up when the frontier's
-// edges exceed the unexplored edges / alpha; back when the frontier shrinks below
-// n / beta).  Levels are unique BFS distances, so every schedule gives the same result.
+// Level L (frontier F = vertices at distance L-1):
+//   expand  every block derives the same direction from the previous level's counts;
+//           top-down: each wave takes 64 frontier vsets, scans their quad counts
+//             across the wave and streams the union of their quads 64 at a time
+//             (every lane busy, loads coalesced); a slice hits when
+//             (mask & frontier byte) != 0; rows not yet visited are OR-ed into
+//             `next` (64-bit atomics);
+//           bottom-up: wave per 64-vertex word, lane per vertex: an unvisited vertex
+//             scans its in-neighbours until one is in the frontier; the wave's
+//             ballot is the new frontier word (no atomics).
+//   update  top-down: thread per bitmap word folds `next` into visited / frontier /
+//             levels and appends the vsets of rsets with new bits to the next list;
+//           bottom-up: the list is rebuilt from the new frontier bitmap.
+// Direction: top-down -> bottom-up when the frontier's edges exceed the unexplored
+// edges / alpha; back when the frontier shrinks below n / beta.  Levels are unique
+// BFS distances, so every schedule produces the same result.
 #include <cstring>
-#include <map>
 
 #include "dgraph.hpp"
+#include "level_common.hpp"
 
 namespace blest {
 namespace dev {
@@ -41,7 +45,6 @@ struct BfsArgs {
     uint32_t src;
     int direction;
     double alpha, beta;
-    unsigned long long timeout;
 };
 
 __device__ __forceinline__ uint32_t front_byte(const unsigned long long* f, uint32_t r) {
@@ -77,215 +80,173 @@ __device__ __forceinline__ void enqueue_words(const GraphView& g, uint32_t w, un
     }
 }
 
-__device__ __forceinline__ void flush_block(Ctrl* c, unsigned slot, unsigned* s_new,
-                                            unsigned long long* s_edges) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (*s_new) atomicAdd(&c->newCount[slot], *s_new);
-        if (*s_edges) atomicAdd(&c->frontEdges[slot], *s_edges);
-        *s_new = 0;
-        *s_edges = 0;
-    }
-}
-
 __device__ __forceinline__ void expand_slice(uint32_t row, uint32_t hit, const unsigned long long* visited,
                                              unsigned long long* next) {
     if (hit && !test_bit(visited, row)) atomicOr(&next[row >> 6], 1ull << (row & 63));
 }
 
-__global__ void __launch_bounds__(1024) bfs_kernel(GraphView g, BfsArgs a) {
-    __shared__ unsigned s_new;
-    __shared__ unsigned long long s_edges;
-    Ctrl* c = a.ctrl;
-    const unsigned lane = lane_id();
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned nthreads = gridDim.x * blockDim.x;
-    const unsigned wave = tid / kWave, nwaves = nthreads / kWave;
-    const uint32_t W = g.W;
-    const bool canBU = g.csrPtr != nullptr;
-    unsigned gen = 0;
-    if (threadIdx.x == 0) {
-        gen = ld_acquire(&c->barGen);
-        s_new = 0;
-        s_edges = 0;
-    }
-    // ---- initialisation (one pass, then one barrier)
+__global__ void bfs_init_kernel(GraphView g, BfsArgs a) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     const uint32_t sw = a.src >> 6;
     const unsigned long long sb = 1ull << (a.src & 63);
-    for (uint32_t w = tid; w < W; w += nthreads) {
+    for (uint32_t w = tid; w < g.W; w += nthreads) {
         const unsigned long long x = (w == sw) ? sb : 0ull;
         a.visited[w] = x;
         a.front1[w] = x;
         a.next[w] = 0ull;
     }
     if (blockIdx.x == 0 && threadIdx.x < kWave) {
+        const unsigned lane = lane_id();
         const uint32_t r = a.src >> 3;
         const uint32_t u0 = g.rsetUnit[r], u1 = g.rsetUnit[r + 1];
         for (uint32_t u = u0 + lane; u < u1; u += kWave) a.list1[u - u0] = u;
         if (lane == 0) {
+            Ctrl* c = a.ctrl;
             a.levels[a.src] = 0;
-            const unsigned long long d = canBU ? (g.csrPtr[a.src + 1] - g.csrPtr[a.src]) : 0ull;
-            for (int s = 0; s < 3; ++s) {
-                c->listCount[s] = 0;
-                c->newCount[s] = 0;
-                c->frontEdges[s] = 0;
-            }
+            level_state_reset(c);
             c->listCount[1] = u1 - u0;
             c->newCount[1] = 1;
-            c->frontEdges[1] = d;
+            c->frontEdges[1] = g.csrPtr ? (g.csrPtr[a.src + 1] - g.csrPtr[a.src]) : 0ull;
         }
     }
-    if (!grid_sync(c, gen, a.timeout)) return;
+}
 
-    unsigned long long visitedEdges = 0;
-    unsigned visitedCount = 0, prevNf = 0, td = 0, bu = 0;
-    bool bottomUp = false, listValid = true;
-    unsigned L = 1;
-    for (;; ++L) {
-        const unsigned sIn = L % 3, sOut = (L + 1) % 3, sRst = (L + 2) % 3;
-        const unsigned nf = ld_relaxed(&c->newCount[sIn]);
-        const unsigned long long mf = ld_relaxed64(&c->frontEdges[sIn]);
-        if (nf == 0) break;
-        visitedCount += nf;
-        visitedEdges += mf;
-        if (a.direction == 0 && canBU) {
-            const double mu = double(g.nnz - min(visitedEdges, g.nnz));
-            if (!bottomUp) {
-                if (double(mf) * a.alpha > mu) bottomUp = true;
-            } else if (double(nf) * a.beta < double(g.n) && nf < prevNf) {
-                bottomUp = false;
+__global__ void __launch_bounds__(256) bfs_expand_kernel(GraphView g, BfsArgs a, unsigned L) {
+    __shared__ BlockCounts sc;
+    DirectionParams dp;
+    dp.direction = a.direction;
+    dp.canBottomUp = g.csrPtr != nullptr;
+    dp.alpha = a.alpha;
+    dp.beta = a.beta;
+    dp.totalEdges = g.nnz;
+    dp.totalItems = double(g.n);
+    Ctrl* c = a.ctrl;
+    bool bottomUp = false;
+    if (!level_begin(c, L, dp, bottomUp)) return;
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned nthreads = gridDim.x * blockDim.x;
+    const unsigned wave = tid / kWave, nwaves = nthreads / kWave;
+    const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
+    unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    if (!bottomUp) {
+        const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
+        const unsigned cnt = ld_relaxed(&c->listCount[L % 3]);
+        for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
+            const unsigned e = base + lane;
+            uint32_t q0 = 0, qc = 0, fb = 0;
+            if (e < cnt) {
+                const uint32_t v = lin[e];
+                q0 = g.unitQuad[v];
+                qc = g.unitQuad[v + 1] - q0;
+                fb = front_byte(fin, g.unitRset[v]);
+                if (!fb) qc = 0;
             }
-        } else {
-            bottomUp = (a.direction == 2 && canBU && L > 1);
+            const unsigned incl = wave_inclusive_scan(qc);
+            const unsigned excl = incl - qc;
+            const unsigned total = __shfl(incl, kWave - 1, kWave);
+            for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+                const unsigned t = t0 + lane;
+                const unsigned owner = wave_owner(excl, min(t, total - 1));
+                const uint32_t oq0 = __shfl(q0, owner, kWave);
+                const unsigned oex = __shfl(excl, owner, kWave);
+                const uint32_t of = __shfl(fb, owner, kWave);
+                if (t < total) {
+                    const uint32_t q = oq0 + (t - oex);
+                    const uint4 r4 = g.rows[q];
+                    const uint32_t m = g.masks[q];
+                    expand_slice(r4.x, m & of, a.visited, a.next);
+                    expand_slice(r4.y, (m >> 8) & of, a.visited, a.next);
+                    expand_slice(r4.z, (m >> 16) & of, a.visited, a.next);
+                    expand_slice(r4.w, (m >> 24) & of, a.visited, a.next);
+                }
+            }
         }
-        prevNf = nf;
-        const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
-        unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
-        uint32_t* lin = (L & 1) ? a.list1 : a.list0;
-        uint32_t* lout = (L & 1) ? a.list0 : a.list1;
-        if (tid == 0) {
-            __hip_atomic_store(&c->listCount[sRst], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&c->newCount[sRst], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&c->frontEdges[sRst], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (!bottomUp) {
-            if (!listValid) {  // frontier came from a bottom-up level: build its vset list
-                for (uint32_t w0 = wave * kWave; w0 < W; w0 += nwaves * kWave) {
-                    const uint32_t w = w0 + lane;
-                    const unsigned long long nw = (w < W) ? fin[w] : 0ull;
-                    enqueue_words(g, w, nw, lin, &c->listCount[sIn]);
-                }
-                if (!grid_sync(c, gen, a.timeout)) return;
-                listValid = true;
-            }
-            const unsigned cnt = ld_relaxed(&c->listCount[sIn]);
-            // ---- phase A: load-balanced expansion of 64 vsets per wave step
-            for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
-                const unsigned e = base + lane;
-                uint32_t q0 = 0, qc = 0, fb = 0;
-                if (e < cnt) {
-                    const uint32_t v = lin[e];
-                    q0 = g.unitQuad[v];
-                    qc = g.unitQuad[v + 1] - q0;
-                    fb = front_byte(fin, g.unitRset[v]);
-                    if (!fb) qc = 0;
-                }
-                const unsigned incl = wave_inclusive_scan(qc);
-                const unsigned excl = incl - qc;
-                const unsigned total = __shfl(incl, kWave - 1, kWave);
-                for (unsigned t0 = 0; t0 < total; t0 += kWave) {
-                    const unsigned t = t0 + lane;
-                    const unsigned owner = wave_owner(excl, min(t, total - 1));
-                    const uint32_t oq0 = __shfl(q0, owner, kWave);
-                    const unsigned oex = __shfl(excl, owner, kWave);
-                    const uint32_t of = __shfl(fb, owner, kWave);
-                    if (t < total) {
-                        const uint32_t q = oq0 + (t - oex);
-                        const uint4 r4 = g.rows[q];
-                        const uint32_t m = g.masks[q];
-                        expand_slice(r4.x, m & of, a.visited, a.next);
-                        expand_slice(r4.y, (m >> 8) & of, a.visited, a.next);
-                        expand_slice(r4.z, (m >> 16) & of, a.visited, a.next);
-                        expand_slice(r4.w, (m >> 24) & of, a.visited, a.next);
-                    }
-                }
-            }
-            if (!grid_sync(c, gen, a.timeout)) return;
-            // ---- phase B: fold `next` into visited / frontier / levels / next list
-            for (uint32_t w0 = wave * kWave; w0 < W; w0 += nwaves * kWave) {
-                const uint32_t w = w0 + lane;
-                unsigned long long nw = 0;
-                if (w < W) {
-                    nw = a.next[w];
-                    if (nw) {
-                        a.next[w] = 0ull;
-                        a.visited[w] |= nw;
-                    }
-                    fout[w] = nw;
-                }
-                unsigned long long edges = 0;
-                for (unsigned long long bits = nw; bits; bits &= bits - 1) {
-                    const uint32_t v = w * 64 + __builtin_ctzll(bits);
-                    a.levels[v] = int32_t(L);
-                    if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
-                }
-                enqueue_words(g, w, nw, lout, &c->listCount[sOut]);
-                const unsigned newc = wave_sum_u32(unsigned(__popcll(nw)));
-                edges = wave_sum_u64(edges);
-                if (lane == 0 && newc) {
-                    atomicAdd(&s_new, newc);
-                    atomicAdd(&s_edges, edges);
-                }
-            }
-            flush_block(c, sOut, &s_new, &s_edges);
-            if (!grid_sync(c, gen, a.timeout)) return;
-            ++td;
-        } else {
-            // ---- bottom-up: wave per word, lane per vertex
-            for (uint32_t w = wave; w < W; w += nwaves) {
-                const unsigned long long vis = a.visited[w];
-                const uint32_t u = w * 64 + lane;
-                bool found = false;
-                uint32_t deg = 0;
-                if (u < g.n && !((vis >> lane) & 1ull)) {
-                    const uint32_t p0 = g.csrPtr[u], p1 = g.csrPtr[u + 1];
-                    deg = p1 - p0;
-                    for (uint32_t p = p0; p < p1; ++p)
-                        if (test_bit(fin, g.csrIdx[p])) {
-                            found = true;
-                            break;
-                        }
-                }
-                const unsigned long long nw = __ballot(found);
-                if (lane == 0) {
-                    if (nw) a.visited[w] = vis | nw;
-                    fout[w] = nw;
-                }
-                if (found) a.levels[u] = int32_t(L);
-                const unsigned long long edges = wave_sum_u64(found ? deg : 0u);
-                if (lane == 0 && nw) {
-                    atomicAdd(&s_new, unsigned(__popcll(nw)));
-                    atomicAdd(&s_edges, edges);
-                }
-            }
-            listValid = false;
-            flush_block(c, sOut, &s_new, &s_edges);
-            if (!grid_sync(c, gen, a.timeout)) return;
-            ++bu;
-        }
+        return;
     }
-    // ---- unreached vertices get -1 (lane per vertex: coalesced stores)
-    for (uint32_t w = wave; w < W; w += nwaves) {
+    // bottom-up: wave per word, lane per vertex
+    block_counts_init(&sc);
+    for (uint32_t w = wave; w < g.W; w += nwaves) {
+        const unsigned long long vis = a.visited[w];
+        const uint32_t u = w * 64 + lane;
+        bool found = false;
+        uint32_t deg = 0;
+        if (u < g.n && !((vis >> lane) & 1ull)) {
+            const uint32_t p0 = g.csrPtr[u], p1 = g.csrPtr[u + 1];
+            deg = p1 - p0;
+            for (uint32_t p = p0; p < p1; ++p)
+                if (test_bit(fin, g.csrIdx[p])) {
+                    found = true;
+                    break;
+                }
+        }
+        const unsigned long long nw = __ballot(found);
+        if (lane == 0) {
+            if (nw) a.visited[w] = vis | nw;
+            fout[w] = nw;
+        }
+        if (found) a.levels[u] = int32_t(L);
+        const unsigned long long edges = wave_sum_u64(found ? deg : 0u);
+        block_counts_add(&sc, unsigned(__popcll(nw)), edges);
+    }
+    const unsigned sOut = (L + 1) % 3;
+    block_counts_flush(&sc, &c->newCount[sOut], &c->frontEdges[sOut]);
+}
+
+__global__ void __launch_bounds__(256) bfs_update_kernel(GraphView g, BfsArgs a, unsigned L) {
+    __shared__ BlockCounts sc;
+    Ctrl* c = a.ctrl;
+    if (ld_relaxed(&c->done)) return;
+    const bool bottomUp = c->state[L & 1].dir != 0;
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned nthreads = gridDim.x * blockDim.x;
+    const unsigned wave = tid / kWave, nwaves = nthreads / kWave;
+    unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    uint32_t* lout = (L & 1) ? a.list0 : a.list1;
+    const unsigned sOut = (L + 1) % 3;
+    if (bottomUp) {  // the next frontier's vset list, from its bitmap
+        for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
+            const uint32_t w = w0 + lane;
+            const unsigned long long nw = (w < g.W) ? fout[w] : 0ull;
+            enqueue_words(g, w, nw, lout, &c->listCount[sOut]);
+        }
+        return;
+    }
+    block_counts_init(&sc);
+    const bool canBU = g.csrPtr != nullptr;
+    for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
+        const uint32_t w = w0 + lane;
+        unsigned long long nw = 0;
+        if (w < g.W) {
+            nw = a.next[w];
+            if (nw) {
+                a.next[w] = 0ull;
+                a.visited[w] |= nw;
+            }
+            fout[w] = nw;
+        }
+        unsigned long long edges = 0;
+        for (unsigned long long bits = nw; bits; bits &= bits - 1) {
+            const uint32_t v = w * 64 + __builtin_ctzll(bits);
+            a.levels[v] = int32_t(L);
+            if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
+        }
+        enqueue_words(g, w, nw, lout, &c->listCount[sOut]);
+        block_counts_add(&sc, wave_sum_u32(unsigned(__popcll(nw))), wave_sum_u64(edges));
+    }
+    block_counts_flush(&sc, &c->newCount[sOut], &c->frontEdges[sOut]);
+}
+
+// Unreached vertices get -1 (lane per vertex: coalesced stores).
+__global__ void __launch_bounds__(256) bfs_final_kernel(GraphView g, BfsArgs a) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
+    const unsigned lane = lane_id();
+    for (uint32_t w = wave; w < g.W; w += nwaves) {
         const unsigned long long vis = a.visited[w];
         const uint32_t u = w * 64 + lane;
         if (u < g.n && !((vis >> lane) & 1ull)) a.levels[u] = -1;
-    }
-    if (tid == 0) {
-        c->levels = L - 1;
-        c->tdLevels = td;
-        c->buLevels = bu;
-        c->visited = visitedCount;
-        c->visitedEdges = visitedEdges;
     }
 }
 
@@ -304,45 +265,33 @@ DeviceGraph::~DeviceGraph() {
 Query::~Query() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : allocations) (void)hipFree(p);
+    if (hostDone) (void)hipHostFree(hostDone);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (evChunk) (void)hipEventDestroy(evChunk);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-unsigned long long timeout_ticks(const DeviceGraph& g) {
-    return (unsigned long long)(double(g.opts.timeout_ms) * g.ticksPerMs);
+dim3 level_grid(const DeviceGraph& g) { return dim3(uint32_t(g.cus) * g.blocksPerCu); }
+
+void run_levels(Query& q, const std::function<void(unsigned)>& level) {
+    unsigned L = 1;
+    unsigned chunk = std::max(4u, q.levelHint + 2);
+    const unsigned limit = q.g->n + 2;
+    for (;;) {
+        for (unsigned i = 0; i < chunk && L <= limit; ++i) level(L++);
+        BLEST_HIP(hipMemcpyAsync(q.hostDone, &q.ctrl->done, sizeof(unsigned), hipMemcpyDeviceToHost, q.stream));
+        BLEST_HIP(hipEventRecord(q.evChunk, q.stream));
+        BLEST_HIP(hipEventSynchronize(q.evChunk));
+        if (*q.hostDone) break;
+        if (L > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
+        chunk = std::min(chunk * 2, 4096u);
+    }
 }
 
-void launch_cooperative(const void* kernel, const DeviceGraph& g, void** args, hipStream_t s) {
-    static std::mutex mu;
-    static std::map<std::pair<const void*, uint32_t>, int> occ;
-    int perCu = 0;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto key = std::make_pair(kernel, g.blockSize);
-        auto it = occ.find(key);
-        if (it == occ.end()) {
-            BLEST_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, int(g.blockSize), 0));
-            occ[key] = perCu;
-        } else {
-            perCu = it->second;
-        }
-    }
-    if (perCu < 1) throw Error(BLEST_ERR_DEVICE, "kernel does not fit on a CU at this block size");
-    uint32_t bpc = g.opts.blocks_per_cu ? std::min<uint32_t>(g.opts.blocks_per_cu, uint32_t(perCu)) : uint32_t(perCu);
-    const uint32_t blocks = uint32_t(g.cus) * bpc;
-    BLEST_HIP(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(g.blockSize), args, 0, s));
-}
-
-void check_ctrl_error(Query& q) {
-    unsigned err = 0;
-    BLEST_HIP(hipMemcpyAsync(&err, &q.ctrl->error, sizeof(err), hipMemcpyDeviceToHost, q.stream));
-    BLEST_HIP(hipStreamSynchronize(q.stream));
-    if (err) {
-        BLEST_HIP(hipMemsetAsync(q.ctrl, 0, sizeof(dev::Ctrl), q.stream));
-        BLEST_HIP(hipStreamSynchronize(q.stream));
-        throw Error(BLEST_ERR_TIMEOUT, "a grid barrier wait expired (workgroups not co-resident?)");
-    }
+uint32_t map_source(const DeviceGraph& g, uint32_t src) {
+    if (src >= g.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "source vertex out of range");
+    return g.hasPerm ? g.hostOldToNew[src] : src;
 }
 
 void bfs_launch(Query& q, uint32_t src) {
@@ -360,11 +309,16 @@ void bfs_launch(Query& q, uint32_t src) {
     a.direction = g.hasCsr ? g.opts.direction : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
-    a.timeout = timeout_ticks(g);
-    GraphView gv = g.view();
-    void* args[] = {&gv, &a};
+    const GraphView gv = g.view();
+    const dim3 grid = level_grid(g), block(256);
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
-    launch_cooperative(reinterpret_cast<const void*>(&dev::bfs_kernel), g, args, q.stream);
+    hipLaunchKernelGGL(dev::bfs_init_kernel, grid, block, 0, q.stream, gv, a);
+    run_levels(q, [&](unsigned L) {
+        hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, a, L);
+        hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, a, L);
+    });
+    hipLaunchKernelGGL(dev::bfs_final_kernel, grid, block, 0, q.stream, gv, a);
+    BLEST_HIP(hipGetLastError());
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     q.lastSource = src;
     q.pending = true;
@@ -374,9 +328,9 @@ void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st) {
     DeviceGraph& g = *q.g;
     BLEST_HIP(hipStreamSynchronize(q.stream));
     q.pending = false;
-    check_ctrl_error(q);
     dev::Ctrl c;
     BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
+    q.levelHint = c.levels;
     float ms = 0;
     BLEST_HIP(hipEventElapsedTime(&ms, q.ev0, q.ev1));
     if (st) {
@@ -386,6 +340,7 @@ void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st) {
         st->td_levels = c.tdLevels;
         st->bu_levels = c.buLevels;
         st->edges_traversed_component = c.visitedEdges / 2;
+        st->bottom_up_mask = c.buMask;
     }
     if (hostLevels) {
         const int32_t* src = q.levels;
@@ -409,8 +364,8 @@ extern "C" {
 void blest_device_opts_default(blest_device_opts* o) {
     if (!o) return;
     o->device = 0;
-    o->block_size = 512;
-    o->blocks_per_cu = 2;
+    o->block_size = 256;
+    o->blocks_per_cu = 8;
     o->alpha = 15.0;
     o->beta = 18.0;
     o->direction = 0;
@@ -429,8 +384,8 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
         blest_device_opts opts;
         blest_device_opts_default(&opts);
         if (o) opts = *o;
-        if (opts.block_size < 64 || opts.block_size > 1024 || opts.block_size % 64)
-            throw Error(BLEST_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 1024]");
+        if (opts.blocks_per_cu == 0 || opts.blocks_per_cu > 64)
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
         if (opts.keep_csr && !gh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "keep_csr needs the CSC");
         auto* h = new blest_device_graph();
         DeviceGraph& g = h->g;
@@ -441,10 +396,9 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
             hipDeviceProp_t prop;
             BLEST_HIP(hipGetDeviceProperties(&prop, g.device));
             g.cus = prop.multiProcessorCount;
-            int khz = 0;
-            if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g.device) == hipSuccess && khz > 0)
-                g.ticksPerMs = double(khz);
-            g.blockSize = opts.block_size;
+            g.blockSize = 256;
+            g.blocksPerCu = opts.blocks_per_cu;
+            g.gridBlocks = uint32_t(g.cus) * g.blocksPerCu;
             g.n = b.n;
             g.R = b.noRealSliceSets;
             g.V = b.noVirtualSliceSets;
@@ -472,8 +426,6 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
                 if (c.n != b.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "CSC and BVSS disagree on n");
                 g.nnz = c.nnz();
                 g.symmetric = csc_is_symmetric(c);
-                g.hostDegree.resize(c.n);
-                for (uint32_t v = 0; v < c.n; ++v) g.hostDegree[v] = c.degree(v);
                 if (opts.keep_csr) {
                     // Bottom-up reads in-neighbours: the CSR of the edges j -> i, i.e. the
                     // transpose of the CSC (equal to it for a symmetric graph).
@@ -544,6 +496,8 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
             BLEST_HIP(hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking));
             BLEST_HIP(hipEventCreate(&q.ev0));
             BLEST_HIP(hipEventCreate(&q.ev1));
+            BLEST_HIP(hipEventCreateWithFlags(&q.evChunk, hipEventDisableTiming));
+            BLEST_HIP(hipHostMalloc(reinterpret_cast<void**>(&q.hostDone), sizeof(unsigned), hipHostMallocDefault));
             q.ctrl = q.alloc<dev::Ctrl>(1);
             BLEST_HIP(hipMemset(q.ctrl, 0, sizeof(dev::Ctrl)));
             q.levels = q.alloc<int32_t>(size_t(g.n) * max_sources);
@@ -564,10 +518,6 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
 
 void blest_query_free(blest_query* q) { delete q; }
 
-static uint32_t map_source(const DeviceGraph& g, uint32_t src) {
-    if (src >= g.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "source vertex out of range");
-    return g.hasPerm ? g.hostOldToNew[src] : src;
-}
 
 blest_status blest_bfs(blest_query* qh, uint32_t src, int32_t* levels, blest_bfs_stats* st) {
     return guarded([&] {

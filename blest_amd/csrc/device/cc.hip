@@ -12,7 +12,7 @@
 //   smaller than its child and the surviving root of a component is its smallest
 //   vertex.  One pass over the slices + one compression pass, independent of diameter.
 // algo 1: min-label propagation, the reference's iteration (frontier = rsets whose
-//   labels dropped), as one persistent cooperative launch.
+//   labels dropped), two stream-ordered kernels per iteration.
 #include "dgraph.hpp"
 
 namespace blest {
@@ -134,7 +134,6 @@ struct LpArgs {
     uint32_t* list0;
     uint32_t* list1;
     Ctrl* ctrl;
-    unsigned long long timeout;
 };
 
 __device__ __forceinline__ void lp_slice(uint32_t* labels, unsigned long long* marker, uint32_t row, uint32_t valid,
@@ -169,85 +168,100 @@ __device__ __forceinline__ void lp_enqueue(const GraphView& g, uint32_t w, unsig
                 for (uint32_t u = g.rsetUnit[w * 8 + b]; u < g.rsetUnit[w * 8 + b + 1]; ++u) list[pos++] = u;
 }
 
-__global__ void __launch_bounds__(1024) cc_lp_kernel(GraphView g, LpArgs a) {
-    Ctrl* c = a.ctrl;
-    const unsigned lane = lane_id();
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned nthreads = gridDim.x * blockDim.x;
-    const unsigned wave = tid / kWave, nwaves = nthreads / kWave;
-    unsigned gen = 0;
-    if (threadIdx.x == 0) gen = ld_acquire(&c->barGen);
-    // All vertices start active; every vset is in the first list.
+// All vertices start active; every vset is in the first list (CC/BVSSCCKernel.cuh:1084-1088,
+// 1183-1203).
+__global__ void __launch_bounds__(256) cc_lp_init_kernel(GraphView g, LpArgs a) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     for (uint32_t i = tid; i < g.n; i += nthreads) a.labels[i] = i;
     for (uint32_t w = tid; w < g.W; w += nthreads) {
-        const uint32_t lo = w * 64;
-        const uint32_t cnt = min(64u, g.n - lo);
+        const uint32_t cnt = min(64u, g.n - w * 64);
         a.front1[w] = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
         a.marker[w] = 0ull;
     }
     for (uint32_t v = tid; v < g.V; v += nthreads) a.list1[v] = v;
     if (tid == 0) {
+        Ctrl* c = a.ctrl;
+        c->done = 0;
         for (int s = 0; s < 3; ++s) c->listCount[s] = 0;
         c->listCount[1] = g.V;
     }
-    if (!grid_sync(c, gen, a.timeout)) return;
-    unsigned it = 1;
-    for (;; ++it) {
-        const unsigned sIn = it % 3, sOut = (it + 1) % 3, sRst = (it + 2) % 3;
-        const unsigned cnt = ld_relaxed(&c->listCount[sIn]);
-        if (cnt == 0) break;
-        const unsigned long long* fin = (it & 1) ? a.front1 : a.front0;
-        unsigned long long* fout = (it & 1) ? a.front0 : a.front1;
-        uint32_t* lin = (it & 1) ? a.list1 : a.list0;
-        uint32_t* lout = (it & 1) ? a.list0 : a.list1;
-        if (tid == 0) __hip_atomic_store(&c->listCount[sRst], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
-            const unsigned e = base + lane;
-            uint32_t q0 = 0, qc = 0, fb = 0, r = 0;
-            if (e < cnt) {
-                const uint32_t v = lin[e];
-                q0 = g.unitQuad[v];
-                qc = g.unitQuad[v + 1] - q0;
-                r = g.unitRset[v];
-                fb = reinterpret_cast<const uint8_t*>(fin)[r];
-                if (!fb) qc = 0;
-            }
-            const unsigned incl = wave_inclusive_scan(qc);
-            const unsigned excl = incl - qc;
-            const unsigned total = __shfl(incl, kWave - 1, kWave);
-            for (unsigned t0 = 0; t0 < total; t0 += kWave) {
-                const unsigned t = t0 + lane;
-                const unsigned owner = wave_owner(excl, min(t, total - 1));
-                const uint32_t oq0 = __shfl(q0, owner, kWave);
-                const unsigned oex = __shfl(excl, owner, kWave);
-                const uint32_t of = __shfl(fb, owner, kWave);
-                const uint32_t orr = __shfl(r, owner, kWave);
-                if (t < total) {
-                    const uint32_t q = oq0 + (t - oex);
-                    const uint4 r4 = g.rows[q];
-                    const uint32_t m = g.masks[q];
-                    const uint32_t cb = orr * 8;
-                    lp_slice(a.labels, a.marker, r4.x, m & of & 0xFF, cb);
-                    lp_slice(a.labels, a.marker, r4.y, (m >> 8) & of & 0xFF, cb);
-                    lp_slice(a.labels, a.marker, r4.z, (m >> 16) & of & 0xFF, cb);
-                    lp_slice(a.labels, a.marker, r4.w, (m >> 24) & of & 0xFF, cb);
-                }
-            }
+}
+
+// Iteration L: every slice of a listed vset whose mask meets its rset's active byte
+// lowers its row's label to the smallest active column label (CC/BVSSCCKernel.cuh:67-221).
+__global__ void __launch_bounds__(256) cc_lp_expand_kernel(GraphView g, LpArgs a, unsigned L) {
+    Ctrl* c = a.ctrl;
+    if (ld_relaxed(&c->done)) return;
+    const unsigned cnt = ld_relaxed(&c->listCount[L % 3]);
+    __syncthreads();
+    if (cnt == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c->iterations = L - 1;
+            __hip_atomic_store(&c->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (!grid_sync(c, gen, a.timeout)) return;
-        for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
-            const uint32_t w = w0 + lane;
-            unsigned long long mk = 0;
-            if (w < g.W) {
-                mk = a.marker[w];
-                if (mk) a.marker[w] = 0ull;
-                fout[w] = mk;
-            }
-            lp_enqueue(g, w, mk, lout, &c->listCount[sOut]);
-        }
-        if (!grid_sync(c, gen, a.timeout)) return;
+        return;
     }
-    if (tid == 0) c->iterations = it - 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->listCount[(L + 2) % 3] = 0;
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
+    const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
+    const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
+    for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
+        const unsigned e = base + lane;
+        uint32_t q0 = 0, qc = 0, fb = 0, r = 0;
+        if (e < cnt) {
+            const uint32_t v = lin[e];
+            q0 = g.unitQuad[v];
+            qc = g.unitQuad[v + 1] - q0;
+            r = g.unitRset[v];
+            fb = reinterpret_cast<const uint8_t*>(fin)[r];
+            if (!fb) qc = 0;
+        }
+        const unsigned incl = wave_inclusive_scan(qc);
+        const unsigned excl = incl - qc;
+        const unsigned total = __shfl(incl, kWave - 1, kWave);
+        for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+            const unsigned t = t0 + lane;
+            const unsigned owner = wave_owner(excl, min(t, total - 1));
+            const uint32_t oq0 = __shfl(q0, owner, kWave);
+            const unsigned oex = __shfl(excl, owner, kWave);
+            const uint32_t of = __shfl(fb, owner, kWave);
+            const uint32_t orr = __shfl(r, owner, kWave);
+            if (t < total) {
+                const uint32_t q = oq0 + (t - oex);
+                const uint4 r4 = g.rows[q];
+                const uint32_t m = g.masks[q];
+                const uint32_t cb = orr * 8;
+                lp_slice(a.labels, a.marker, r4.x, m & of & 0xFF, cb);
+                lp_slice(a.labels, a.marker, r4.y, (m >> 8) & of & 0xFF, cb);
+                lp_slice(a.labels, a.marker, r4.z, (m >> 16) & of & 0xFF, cb);
+                lp_slice(a.labels, a.marker, r4.w, (m >> 24) & of & 0xFF, cb);
+            }
+        }
+    }
+}
+
+// Marked vertices become the next active set; their rsets' vsets the next list
+// (CC/BVSSCCKernel.cuh:228-269).
+__global__ void __launch_bounds__(256) cc_lp_update_kernel(GraphView g, LpArgs a, unsigned L) {
+    Ctrl* c = a.ctrl;
+    if (ld_relaxed(&c->done)) return;
+    const unsigned lane = lane_id();
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
+    unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    uint32_t* lout = (L & 1) ? a.list0 : a.list1;
+    for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
+        const uint32_t w = w0 + lane;
+        unsigned long long mk = 0;
+        if (w < g.W) {
+            mk = a.marker[w];
+            if (mk) a.marker[w] = 0ull;
+            fout[w] = mk;
+        }
+        lp_enqueue(g, w, mk, lout, &c->listCount[(L + 1) % 3]);
+    }
 }
 
 }  // namespace dev
@@ -278,16 +292,19 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
         a.list0 = q.list0;
         a.list1 = q.list1;
         a.ctrl = q.ctrl;
-        a.timeout = timeout_ticks(g);
-        void* args[] = {&gv, &a};
-        launch_cooperative(reinterpret_cast<const void*>(&dev::cc_lp_kernel), g, args, q.stream);
+        const dim3 lg = level_grid(g);
+        hipLaunchKernelGGL(dev::cc_lp_init_kernel, lg, block, 0, q.stream, gv, a);
+        run_levels(q, [&](unsigned L) {
+            hipLaunchKernelGGL(dev::cc_lp_expand_kernel, lg, block, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::cc_lp_update_kernel, lg, block, 0, q.stream, gv, a, L);
+        });
+        BLEST_HIP(hipGetLastError());
     } else {
         throw Error(BLEST_ERR_INVALID_ARGUMENT, "unknown CC algorithm");
     }
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     BLEST_HIP(hipStreamSynchronize(q.stream));
     if (algo == 1) {
-        check_ctrl_error(q);
         dev::Ctrl c;
         BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
         iterations = c.iterations;

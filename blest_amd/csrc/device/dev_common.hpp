@@ -9,23 +9,34 @@ namespace dev {
 
 constexpr int kWave = 64;
 
-// Control block of one persistent traversal (device memory, zeroed per launch).
-// Counters are rotated over three slots indexed by level % 3: the slot a level
-// writes is reset two levels earlier, so no extra grid barrier is needed to clear it.
+// Direction and running totals of a level-synchronous traversal.  Level L's expand
+// kernel reads slot (L-1)&1 and its block 0 writes slot L&1, so readers and the
+// writer never share a slot within one kernel.
+struct LevelState {
+    unsigned dir;        // 0 top-down, 1 bottom-up (the level that wrote the slot)
+    unsigned prevNf;     // frontier size that level consumed
+    unsigned visited;    // vertices (or vertex-root pairs) reached before that level
+    unsigned td, bu;     // levels run in each direction
+    unsigned pad;
+    unsigned long long cumEdges;  // degree sum of the vertices reached before that level
+    unsigned long long buMask;    // bit l-1: level l ran bottom-up (l <= 64)
+};
+
+// Control block of one traversal (device memory).  Counters are rotated over three
+// slots indexed by level % 3: level L reads slot L%3 (what level L-1 produced),
+// writes slot (L+1)%3 and clears slot (L+2)%3, which nobody touches during L.
 struct alignas(256) Ctrl {
-    unsigned barCount;   // grid barrier: arrivals in the current round
-    unsigned barGen;     // grid barrier: completed rounds
-    unsigned error;      // nonzero: a bounded wait expired; every block leaves
-    unsigned done;
+    unsigned done;                // the traversal has finished (set by an expand kernel)
+    unsigned levels;              // number of levels (max distance + 1)
+    unsigned tdLevels, buLevels;
+    unsigned visited;
+    unsigned iterations;
     unsigned listCount[3];        // frontier list length (vsets) per slot
     unsigned newCount[3];         // vertices (or vertex-root pairs) discovered per slot
-    unsigned long long frontEdges[3];  // sum of degrees of the discovered vertices
-    unsigned levels;              // number of levels run
-    unsigned tdLevels, buLevels;
-    unsigned visited;             // vertices reached (including the source)
+    unsigned long long frontEdges[3];  // degree sum of the discovered vertices per slot
     unsigned long long visitedEdges;
-    unsigned iterations;
-    unsigned pad[13];
+    unsigned long long buMask;
+    LevelState state[2];
 };
 
 // ------------------------------------------------------------------ wave helpers
@@ -54,7 +65,7 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned x) {
 }
 
 // Largest lane j whose value excl[j] <= t, given excl ascending over lanes with
-// excl[0] == 0 (owner search of the load-balanced expansion).
+// excl[0] == 0 (owner search of the load-balanced expansion).  All lanes must call.
 __device__ __forceinline__ unsigned wave_owner(unsigned excl, unsigned t) {
     unsigned lo = 0;
 #pragma unroll
@@ -65,11 +76,6 @@ __device__ __forceinline__ unsigned wave_owner(unsigned excl, unsigned t) {
     return lo;
 }
 
-__device__ __forceinline__ unsigned long long now_ticks() { return wall_clock64(); }
-
-__device__ __forceinline__ unsigned ld_acquire(const unsigned* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ unsigned ld_relaxed(const unsigned* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -77,45 +83,38 @@ __device__ __forceinline__ unsigned long long ld_relaxed64(const unsigned long l
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Grid-wide barrier for a co-resident grid (launched cooperatively).  Every wait is
-// bounded by `timeout` wall-clock ticks: on expiry the block raises ctrl->error and
-// leaves; blocks waiting elsewhere see the flag and leave too, so the kernel always
-// drains.  `gen` is the number of rounds completed so far (meaningful in thread 0).
-// Returns false when the traversal must stop because of an error.
-__device__ __forceinline__ bool grid_sync(Ctrl* c, unsigned& gen, unsigned long long timeout) {
-    __shared__ unsigned s_ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned ok = 1;
-        __threadfence();
-        const unsigned nb = gridDim.x;
-        unsigned arrived = atomicAdd(&c->barCount, 1u);
-        if (arrived == nb - 1) {
-            atomicExch(&c->barCount, 0u);
-            __threadfence();
-            __hip_atomic_store(&c->barGen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            const unsigned long long deadline = now_ticks() + timeout;
-            while (ld_acquire(&c->barGen) == gen) {
-                if (ld_relaxed(&c->error)) break;
-                if (now_ticks() > deadline) {
-                    atomicOr(&c->error, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        ++gen;
-        __threadfence();
-        if (ld_relaxed(&c->error)) ok = 0;
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
-
 __device__ __forceinline__ bool test_bit(const unsigned long long* bm, unsigned v) {
     return (bm[v >> 6] >> (v & 63)) & 1ull;
+}
+
+// Block-level aggregation of per-wave counts, flushed with one global atomic per
+// counter per block at the end of the kernel.
+struct BlockCounts {
+    unsigned n;
+    unsigned long long e;
+};
+
+__device__ __forceinline__ void block_counts_init(BlockCounts* s) {
+    if (threadIdx.x == 0) {
+        s->n = 0;
+        s->e = 0;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void block_counts_add(BlockCounts* s, unsigned n, unsigned long long e) {
+    if (lane_id() == 0 && n) {
+        atomicAdd(&s->n, n);
+        atomicAdd(&s->e, e);
+    }
+}
+
+__device__ __forceinline__ void block_counts_flush(BlockCounts* s, unsigned* n, unsigned long long* e) {
+    __syncthreads();
+    if (threadIdx.x == 0 && s->n) {
+        atomicAdd(n, s->n);
+        atomicAdd(e, s->e);
+    }
 }
 
 }  // namespace dev

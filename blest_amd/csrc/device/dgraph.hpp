@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -45,8 +46,7 @@ struct GraphView {
 struct DeviceGraph {
     int device = 0;
     int cus = 0;
-    uint32_t blockSize = 512, gridBlocks = 0;
-    double ticksPerMs = 100000.0;  // wall_clock64 frequency
+    uint32_t blockSize = 256, blocksPerCu = 8, gridBlocks = 0;
     blest_device_opts opts{};
     uint32_t n = 0, R = 0, V = 0, W = 0;
     uint64_t S = 0, Q = 0, nnz = 0;
@@ -92,7 +92,9 @@ struct Query {
     DeviceGraph* g = nullptr;
     uint32_t maxSources = 1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evChunk = nullptr;
+    unsigned* hostDone = nullptr;  // pinned: the traversal's done flag, polled per chunk
+    unsigned levelHint = 0;        // levels of the last traversal (sizes the first chunk)
     dev::Ctrl* ctrl = nullptr;
     int32_t* levels = nullptr;  // n * maxSources
     unsigned long long *visited = nullptr, *front0 = nullptr, *front1 = nullptr, *next = nullptr;
@@ -103,6 +105,7 @@ struct Query {
     // multi-source state, allocated on first use: words per vertex x n
     uint32_t mbfsWords = 0;
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
+    uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     std::vector<void*> allocations;
     std::mutex mu;
     uint32_t lastSource = 0;
@@ -118,10 +121,11 @@ struct Query {
     ~Query();
 };
 
-// Launch helpers (graph.hip).
-void launch_cooperative(const void* kernel, const DeviceGraph& g, void** args, hipStream_t s);
-void check_ctrl_error(Query& q);
-unsigned long long timeout_ticks(const DeviceGraph& g);
+// Level-loop driver (bfs.hip): enqueues level(L) for L = 1, 2, ... in chunks and polls
+// the device's done flag after each chunk.
+dim3 level_grid(const DeviceGraph& g);
+void run_levels(Query& q, const std::function<void(unsigned)>& level);
+uint32_t map_source(const DeviceGraph& g, uint32_t src);
 void bfs_launch(Query& q, uint32_t srcReordered);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);
 

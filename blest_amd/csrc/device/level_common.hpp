@@ -1,0 +1,83 @@
+// Direction decision shared by the level-synchronous traversals (single- and
+// multi-source BFS).  Every block of level L's expand kernel evaluates it from the same
+// inputs, so all blocks agree; block 0 publishes the result in ctrl->state[L & 1].
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace blest {
+namespace dev {
+
+struct DirectionParams {
+    int direction;                  // 0 optimising, 1 top-down only, 2 bottom-up after level 1
+    bool canBottomUp;               // the bottom-up CSR is resident
+    double alpha, beta;
+    unsigned long long totalEdges;  // edges (or edge-root pairs) of the whole graph
+    double totalItems;              // vertices (or vertex-root pairs)
+};
+
+// Returns false (and, in block 0, records the final totals and raises ctrl->done) when
+// the previous level discovered nothing.  Must be called by every thread of the block.
+__device__ __forceinline__ bool level_begin(Ctrl* c, unsigned L, const DirectionParams& p, bool& bottomUp) {
+    if (ld_relaxed(&c->done)) return false;
+    const LevelState in = c->state[(L - 1) & 1];
+    const unsigned sIn = L % 3;
+    const unsigned nf = ld_relaxed(&c->newCount[sIn]);
+    const unsigned long long mf = ld_relaxed64(&c->frontEdges[sIn]);
+    __syncthreads();  // every thread has read state / counters before block 0 writes
+    if (nf == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c->levels = L - 1;
+            c->tdLevels = in.td;
+            c->buLevels = in.bu;
+            c->visited = in.visited;
+            c->visitedEdges = in.cumEdges;
+            c->buMask = in.buMask;
+            __hip_atomic_store(&c->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return false;
+    }
+    const unsigned long long cum = in.cumEdges + mf;
+    bottomUp = in.dir != 0;
+    if (p.direction == 0 && p.canBottomUp) {
+        const double mu = double(p.totalEdges - min(cum, p.totalEdges));
+        if (!bottomUp) {
+            if (double(mf) * p.alpha > mu) bottomUp = true;
+        } else if (double(nf) * p.beta < p.totalItems && nf < in.prevNf) {
+            bottomUp = false;
+        }
+    } else {
+        bottomUp = (p.direction == 2 && p.canBottomUp && L > 1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        LevelState out;
+        out.dir = bottomUp ? 1u : 0u;
+        out.prevNf = nf;
+        out.visited = in.visited + nf;
+        out.td = in.td + (bottomUp ? 0u : 1u);
+        out.bu = in.bu + (bottomUp ? 1u : 0u);
+        out.pad = 0;
+        out.cumEdges = cum;
+        out.buMask = in.buMask | ((bottomUp && L <= 64) ? (1ull << (L - 1)) : 0ull);
+        c->state[L & 1] = out;
+        const unsigned sRst = (L + 2) % 3;
+        c->listCount[sRst] = 0;
+        c->newCount[sRst] = 0;
+        c->frontEdges[sRst] = 0;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void level_state_reset(Ctrl* c) {
+    c->done = 0;
+    for (int s = 0; s < 3; ++s) {
+        c->listCount[s] = 0;
+        c->newCount[s] = 0;
+        c->frontEdges[s] = 0;
+    }
+    c->state[0] = LevelState{0, 0, 0, 0, 0, 0, 0ull, 0ull};
+    c->state[1] = c->state[0];
+}
+
+}  // namespace dev
+}  // namespace blest

@@ -245,14 +245,18 @@ std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uin
 // ---------------------------------------------------------------------- Gorder
 namespace {
 
-// Exact max-priority over integer keys with +-1 updates: one doubly linked list per
-// key and a pointer to the largest non-empty key.
+// Max-priority over integer keys with unit updates: one doubly linked list per key
+// and a pointer to the largest non-empty key.  Increments move the vertex at once;
+// decrements are only counted and applied when the vertex reaches the top (a vertex
+// whose stored key is the largest and has no pending decrements holds the true
+// maximum), so the window-eviction half of the updates costs one counter write.
 class BucketQueue {
 public:
-    explicit BucketQueue(uint32_t n) : key_(n, 0), prev_(n), next_(n), in_(n, 1), head_(1, -1) {
+    explicit BucketQueue(uint32_t n)
+        : key_(n, 0), pend_(n, 0), prev_(n), next_(n), in_(n, 1), head_(16, -1) {
         for (uint32_t v = 0; v < n; ++v) {
-            prev_[v] = int64_t(v) - 1;
-            next_[v] = (v + 1 < n) ? int64_t(v) + 1 : -1;
+            prev_[v] = int32_t(v) - 1;
+            next_[v] = (v + 1 < n) ? int32_t(v) + 1 : -1;
         }
         if (n) head_[0] = 0;
         top_ = 0;
@@ -273,39 +277,47 @@ public:
         if (key_[v] > top_) top_ = key_[v];
     }
     void dec(uint32_t v) {
-        if (!in_[v]) return;
-        unlink(v);
-        --key_[v];
-        link(v);
+        if (in_[v]) ++pend_[v];
     }
-    // Largest key; among equal keys the most recently linked vertex.
+    // Largest effective key; among equal keys the most recently linked vertex.
     int64_t pop_max() {
-        while (top_ > 0 && head_[top_] < 0) --top_;
-        int64_t v = head_[top_];
-        if (v >= 0) remove(uint32_t(v));
-        return v;
+        for (;;) {
+            while (top_ > 0 && head_[top_] < 0) --top_;
+            const int32_t v = head_[top_];
+            if (v < 0) return -1;
+            if (pend_[v]) {
+                unlink(uint32_t(v));
+                key_[v] -= pend_[v];
+                pend_[v] = 0;
+                link(uint32_t(v));
+                continue;
+            }
+            remove(uint32_t(v));
+            return v;
+        }
     }
 
 private:
     void unlink(uint32_t v) {
-        uint64_t k = key_[v];
+        const uint32_t k = key_[v];
         if (prev_[v] >= 0) next_[prev_[v]] = next_[v];
         else head_[k] = next_[v];
         if (next_[v] >= 0) prev_[next_[v]] = prev_[v];
     }
     void link(uint32_t v) {
-        uint64_t k = key_[v];
-        if (k >= head_.size()) head_.resize(std::max<uint64_t>(k + 1, head_.size() * 2), -1);
+        const uint32_t k = key_[v];
+        if (k >= head_.size()) head_.resize(std::max<size_t>(size_t(k) + 1, head_.size() * 2), -1);
         prev_[v] = -1;
         next_[v] = head_[k];
-        if (head_[k] >= 0) prev_[head_[k]] = v;
-        head_[k] = v;
+        if (head_[k] >= 0) prev_[head_[k]] = int32_t(v);
+        head_[k] = int32_t(v);
     }
-    std::vector<uint64_t> key_;
-    std::vector<int64_t> prev_, next_;
+    std::vector<uint32_t> key_, pend_;
+    std::vector<int32_t> prev_, next_;
     std::vector<uint8_t> in_;
-    std::vector<int64_t> head_;
-    uint64_t top_, size_;
+    std::vector<int32_t> head_;
+    uint32_t top_;
+    uint64_t size_;
 };
 
 }  // namespace
@@ -315,9 +327,10 @@ private:
 // vertex maximising the Gorder score S(v, w) = |in(v) & in(w)| + [v->w] + [w->v]
 // summed over the last `window` placed vertices w.  As in the reference, in-neighbours
 // whose out-degree exceeds sqrt(n) contribute no sibling score, the first vertex is
-// the one of largest in-degree, and vertices without edges go last.  Priorities are
-// kept exact (the reference's UnitHeap applies decrements lazily).  Unlike the
-// reference (CSC.cuh:936-943) the order is returned in the caller's vertex IDs.
+// the one of largest in-degree, and vertices without edges go last.  Every extraction
+// returns a vertex of exactly maximal score (the reference's UnitHeap halves pending
+// decrements when it re-files the top).  Unlike the reference (CSC.cuh:936-943) the
+// order is returned in the caller's vertex IDs.
 std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
     const uint32_t n = g.n;
     std::vector<uint32_t> inv(n);

@@ -209,6 +209,7 @@ typedef struct blest_bfs_stats {
     uint32_t no_visited;
     uint32_t td_levels, bu_levels;
     uint64_t edges_traversed_component; /* 1/2 * sum of degrees of visited vertices */
+    uint64_t bottom_up_mask;            /* bit l-1 set: level l ran bottom-up (l <= 64) */
 } blest_bfs_stats;
 
 /* BFSKernel::run(src) (BFS/BFSKernel.cuh:53-56): single-source BFS.  levels[n]
