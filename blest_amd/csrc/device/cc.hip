@@ -134,6 +134,8 @@ struct LpArgs {
     uint32_t* list0;
     uint32_t* list1;
     Ctrl* ctrl;
+    SegView sv;
+    SegCounters sc;
 };
 
 __device__ __forceinline__ void lp_slice(uint32_t* labels, unsigned long long* marker, uint32_t row, uint32_t valid,
@@ -148,29 +150,9 @@ __device__ __forceinline__ void lp_slice(uint32_t* labels, unsigned long long* m
     }
 }
 
-// Wave-cooperative list append (same contract as the BFS one).
-__device__ __forceinline__ void lp_enqueue(const GraphView& g, uint32_t w, unsigned long long nw, uint32_t* list,
-                                           unsigned* counter) {
-    unsigned cnt = 0;
-    if (nw)
-        for (int b = 0; b < 8; ++b)
-            if ((nw >> (8 * b)) & 0xFFull) cnt += g.rsetUnit[w * 8 + b + 1] - g.rsetUnit[w * 8 + b];
-    const unsigned incl = wave_inclusive_scan(cnt);
-    const unsigned total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) return;
-    unsigned base = 0;
-    if (lane_id() == kWave - 1) base = atomicAdd(counter, total);
-    base = __shfl(base, kWave - 1, kWave);
-    unsigned pos = base + incl - cnt;
-    if (cnt)
-        for (int b = 0; b < 8; ++b)
-            if ((nw >> (8 * b)) & 0xFFull)
-                for (uint32_t u = g.rsetUnit[w * 8 + b]; u < g.rsetUnit[w * 8 + b + 1]; ++u) list[pos++] = u;
-}
-
 // All vertices start active; every vset is in the first list (CC/BVSSCCKernel.cuh:1084-1088,
 // 1183-1203).
-__global__ void __launch_bounds__(256) cc_lp_init_kernel(GraphView g, LpArgs a) {
+__global__ void __launch_bounds__(kProducerBlock) cc_lp_init_kernel(GraphView g, LpArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     for (uint32_t i = tid; i < g.n; i += nthreads) a.labels[i] = i;
     for (uint32_t w = tid; w < g.W; w += nthreads) {
@@ -178,22 +160,21 @@ __global__ void __launch_bounds__(256) cc_lp_init_kernel(GraphView g, LpArgs a) 
         a.front1[w] = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
         a.marker[w] = 0ull;
     }
+    // every segment lists all of its vsets
     for (uint32_t v = tid; v < g.V; v += nthreads) a.list1[v] = v;
-    if (tid == 0) {
-        Ctrl* c = a.ctrl;
-        c->done = 0;
-        for (int s = 0; s < 3; ++s) c->listCount[s] = 0;
-        c->listCount[1] = g.V;
-    }
+    for (uint32_t s = tid; s < a.sv.segs; s += nthreads)
+        a.sc.count[a.sv.segs + s] = a.sv.segBase[s + 1] - a.sv.segBase[s];
+    if (tid == 0) a.ctrl->done = 0;
 }
 
 // Iteration L: every slice of a listed vset whose mask meets its rset's active byte
 // lowers its row's label to the smallest active column label (CC/BVSSCCKernel.cuh:67-221).
-__global__ void __launch_bounds__(256) cc_lp_expand_kernel(GraphView g, LpArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) cc_lp_expand_kernel(GraphView g, LpArgs a, unsigned L) {
+    __shared__ unsigned pre[kMaxSegs];
+    __shared__ unsigned tmp[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
-    const unsigned cnt = ld_relaxed(&c->listCount[L % 3]);
-    __syncthreads();
+    const unsigned cnt = load_prefix(a.sc.count + size_t(L % 3) * a.sv.segs, a.sv.segs, pre, tmp);
     if (cnt == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             c->iterations = L - 1;
@@ -201,7 +182,6 @@ __global__ void __launch_bounds__(256) cc_lp_expand_kernel(GraphView g, LpArgs a
         }
         return;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->listCount[(L + 2) % 3] = 0;
     const unsigned lane = lane_id();
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
@@ -211,7 +191,7 @@ __global__ void __launch_bounds__(256) cc_lp_expand_kernel(GraphView g, LpArgs a
         const unsigned e = base + lane;
         uint32_t q0 = 0, qc = 0, fb = 0, r = 0;
         if (e < cnt) {
-            const uint32_t v = lin[e];
+            const uint32_t v = list_entry(lin, a.sv, pre, e);
             q0 = g.unitQuad[v];
             qc = g.unitQuad[v + 1] - q0;
             r = g.unitRset[v];
@@ -244,24 +224,30 @@ __global__ void __launch_bounds__(256) cc_lp_expand_kernel(GraphView g, LpArgs a
 
 // Marked vertices become the next active set; their rsets' vsets the next list
 // (CC/BVSSCCKernel.cuh:228-269).
-__global__ void __launch_bounds__(256) cc_lp_update_kernel(GraphView g, LpArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) cc_lp_update_kernel(GraphView g, LpArgs a, unsigned L) {
+    __shared__ unsigned tmp[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
-    const unsigned lane = lane_id();
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     uint32_t* lout = (L & 1) ? a.list0 : a.list1;
-    for (uint32_t w0 = wave * kWave; w0 < g.W; w0 += nwaves * kWave) {
-        const uint32_t w = w0 + lane;
+    const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
+    uint32_t* out = lout + a.sv.segBase[blockIdx.x];
+    unsigned running = 0;
+    for (uint32_t r0 = ws; r0 < we; r0 += blockDim.x) {
+        const uint32_t w = r0 + threadIdx.x;
         unsigned long long mk = 0;
-        if (w < g.W) {
+        if (w < we) {
             mk = a.marker[w];
             if (mk) a.marker[w] = 0ull;
             fout[w] = mk;
         }
-        lp_enqueue(g, w, mk, lout, &c->listCount[(L + 1) % 3]);
+        const unsigned cnt = word_vsets(g.rsetUnit, w, mk);
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(cnt, tmp, tot);
+        write_word_vsets(g.rsetUnit, w, mk, out + running + ex);
+        running += tot;
     }
+    if (threadIdx.x == 0) a.sc.count[size_t((L + 1) % 3) * a.sv.segs + blockIdx.x] = running;
 }
 
 }  // namespace dev
@@ -292,11 +278,13 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
         a.list0 = q.list0;
         a.list1 = q.list1;
         a.ctrl = q.ctrl;
-        const dim3 lg = level_grid(g);
-        hipLaunchKernelGGL(dev::cc_lp_init_kernel, lg, block, 0, q.stream, gv, a);
+        a.sv = g.seg_view();
+        a.sc = q.seg_counters();
+        const dim3 lg = level_grid(g), lb(dev::kProducerBlock);
+        hipLaunchKernelGGL(dev::cc_lp_init_kernel, lg, lb, 0, q.stream, gv, a);
         run_levels(q, [&](unsigned L) {
-            hipLaunchKernelGGL(dev::cc_lp_expand_kernel, lg, block, 0, q.stream, gv, a, L);
-            hipLaunchKernelGGL(dev::cc_lp_update_kernel, lg, block, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::cc_lp_expand_kernel, lg, lb, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::cc_lp_update_kernel, lg, lb, 0, q.stream, gv, a, L);
         });
         BLEST_HIP(hipGetLastError());
     } else {

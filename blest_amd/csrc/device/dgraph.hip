@@ -1,0 +1,233 @@
+// HBM-resident device graph and per-query state: the C ABI handles
+// blest_device_graph / blest_query, and the host-side level-loop driver.
+#include "dgraph.hpp"
+#include "segments.hpp"
+
+namespace blest {
+
+// ----------------------------------------------------------------- host helpers
+DeviceGraph::~DeviceGraph() {
+    for (void* p : allocations) (void)hipFree(p);
+}
+
+Query::~Query() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocations) (void)hipFree(p);
+    if (hostDone) (void)hipHostFree(hostDone);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (evChunk) (void)hipEventDestroy(evChunk);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+dim3 level_grid(const DeviceGraph& g) { return dim3(g.segs); }
+
+dev::SegView DeviceGraph::seg_view() const { return dev::SegView{segs, wps, segBase}; }
+
+dev::SegCounters Query::seg_counters() const { return dev::SegCounters{segCount, segNew, segEdges}; }
+
+void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail) {
+    unsigned L = 1;
+    unsigned chunk = std::max(4u, q.levelHint + 2);
+    const unsigned limit = q.g->n + 2;
+    for (;;) {
+        for (unsigned i = 0; i < chunk && L <= limit; ++i) level(L++);
+        // The tail (e.g. marking unreached vertices) is idempotent and harmless before
+        // the end: enqueuing it ahead of the poll hides the host round trip when the
+        // chunk was the last one.
+        if (tail) tail();
+        BLEST_HIP(hipMemcpyAsync(q.hostDone, &q.ctrl->done, sizeof(unsigned), hipMemcpyDeviceToHost, q.stream));
+        BLEST_HIP(hipEventRecord(q.evChunk, q.stream));
+        BLEST_HIP(hipEventSynchronize(q.evChunk));
+        if (*q.hostDone) break;
+        if (L > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
+        chunk = std::min(chunk * 2, 4096u);
+    }
+}
+
+uint32_t map_source(const DeviceGraph& g, uint32_t src) {
+    if (src >= g.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "source vertex out of range");
+    return g.hasPerm ? g.hostOldToNew[src] : src;
+}
+
+}  // namespace blest
+
+using namespace blest;
+
+extern "C" {
+
+void blest_device_opts_default(blest_device_opts* o) {
+    if (!o) return;
+    o->device = 0;
+    o->block_size = 256;
+    o->blocks_per_cu = 4;
+    o->alpha = 15.0;
+    o->beta = 18.0;
+    o->direction = 0;
+    o->keep_csr = 1;
+    o->timeout_ms = 20000;
+}
+
+blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,
+                                       const blest_device_opts* o, blest_device_graph** out) {
+    return guarded([&] {
+        if (!bh || !out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        const BVSS& b = bvss_of(bh);
+        if (b.sliceSize != 8 || b.noMasks != 4)
+            throw Error(BLEST_ERR_SLICE_SIZE, "No appropriate kernel found meeting the selected slice size and noMasks.");
+        if (b.n == 0) throw Error(BLEST_ERR_INVALID_ARGUMENT, "empty graph");
+        blest_device_opts opts;
+        blest_device_opts_default(&opts);
+        if (o) opts = *o;
+        if (opts.blocks_per_cu == 0 || opts.blocks_per_cu > 8)
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 8]");
+        if (opts.keep_csr && !gh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "keep_csr needs the CSC");
+        auto* h = new blest_device_graph();
+        DeviceGraph& g = h->g;
+        try {
+            g.opts = opts;
+            g.device = opts.device;
+            BLEST_HIP(hipSetDevice(g.device));
+            hipDeviceProp_t prop;
+            BLEST_HIP(hipGetDeviceProperties(&prop, g.device));
+            g.cus = prop.multiProcessorCount;
+            g.blockSize = dev::kProducerBlock;
+            g.blocksPerCu = opts.blocks_per_cu;
+            g.n = b.n;
+            g.R = b.noRealSliceSets;
+            g.V = b.noVirtualSliceSets;
+            g.W = (b.n + 63) / 64;
+            g.S = b.noSlices;
+            g.Q = b.noSlices / 4;
+            if (g.Q > 0xFFFFFFFFull) throw Error(BLEST_ERR_UNSUPPORTED, "more than 2^32 slice quads");
+            // Segments: one producer workgroup per contiguous run of bitmap words.
+            g.segs = std::min<uint32_t>(std::min<uint32_t>(uint32_t(g.cus) * g.blocksPerCu, dev::kMaxSegs), g.W);
+            g.wps = (g.W + g.segs - 1) / g.segs;
+            g.segs = (g.W + g.wps - 1) / g.wps;
+            g.gridBlocks = g.segs;
+            {
+                std::vector<uint32_t> sb(size_t(g.segs) + 1);
+                for (uint32_t s = 0; s <= g.segs; ++s) {
+                    const uint64_t r = std::min<uint64_t>(uint64_t(s) * g.wps * 8, g.R);
+                    sb[s] = b.realPtrs[r];
+                }
+                g.segBase = g.alloc<uint32_t>(sb.size());
+                BLEST_HIP(hipMemcpy(g.segBase, sb.data(), 4 * sb.size(), hipMemcpyHostToDevice));
+            }
+            std::vector<uint32_t> uq(size_t(g.V) + 1);
+            for (size_t v = 0; v <= g.V; ++v) {
+                if (b.sliceSetPtrs[v] % 4) throw Error(BLEST_ERR_INVALID_ARGUMENT, "vset not a multiple of 4 slices");
+                uq[v] = uint32_t(b.sliceSetPtrs[v] / 4);
+            }
+            g.unitQuad = g.alloc<uint32_t>(uq.size());
+            BLEST_HIP(hipMemcpy(g.unitQuad, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice));
+            g.unitRset = g.alloc<uint32_t>(g.V);
+            BLEST_HIP(hipMemcpy(g.unitRset, b.virtualToReal.data(), 4 * size_t(g.V), hipMemcpyHostToDevice));
+            g.rsetUnit = g.alloc<uint32_t>(size_t(g.R) + 1);
+            BLEST_HIP(hipMemcpy(g.rsetUnit, b.realPtrs.data(), 4 * (size_t(g.R) + 1), hipMemcpyHostToDevice));
+            g.rows = g.alloc<uint4>(g.Q);
+            BLEST_HIP(hipMemcpy(g.rows, b.rowIds.data(), 16 * g.Q, hipMemcpyHostToDevice));
+            g.masks = g.alloc<uint32_t>(g.Q);
+            BLEST_HIP(hipMemcpy(g.masks, b.masks.data(), 4 * g.Q, hipMemcpyHostToDevice));
+            if (gh) {
+                const CSC& c = csc_of(gh);
+                if (c.n != b.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "CSC and BVSS disagree on n");
+                g.nnz = c.nnz();
+                g.symmetric = csc_is_symmetric(c);
+                if (opts.keep_csr) {
+                    // Bottom-up reads in-neighbours: the CSR of the edges j -> i, i.e. the
+                    // transpose of the CSC (equal to it for a symmetric graph).
+                    CSC tr;
+                    const CSC* csr = &c;
+                    if (!g.symmetric) {
+                        tr = csc_transpose(c);
+                        csr = &tr;
+                    }
+                    g.csrPtr = g.alloc<uint32_t>(size_t(c.n) + 1);
+                    BLEST_HIP(hipMemcpy(g.csrPtr, csr->colPtrs.data(), 4 * (size_t(c.n) + 1), hipMemcpyHostToDevice));
+                    g.csrIdx = g.alloc<uint32_t>(csr->nnz());
+                    if (csr->nnz())
+                        BLEST_HIP(hipMemcpy(g.csrIdx, csr->rows.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
+                    g.hasCsr = true;
+                }
+            }
+            if (inv_perm) {
+                if (!permutation_is_bijection(inv_perm, b.n)) throw Error(BLEST_ERR_PERMUTATION, "Permutation check has failed.");
+                g.hostOldToNew.assign(inv_perm, inv_perm + b.n);
+                g.hostNewToOld.resize(b.n);
+                for (uint32_t old = 0; old < b.n; ++old) g.hostNewToOld[inv_perm[old]] = old;
+                g.permOldToNew = g.alloc<uint32_t>(b.n);
+                g.permNewToOld = g.alloc<uint32_t>(b.n);
+                BLEST_HIP(hipMemcpy(g.permOldToNew, g.hostOldToNew.data(), 4 * size_t(b.n), hipMemcpyHostToDevice));
+                BLEST_HIP(hipMemcpy(g.permNewToOld, g.hostNewToOld.data(), 4 * size_t(b.n), hipMemcpyHostToDevice));
+                g.hasPerm = true;
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void blest_device_graph_free(blest_device_graph* dg) { delete dg; }
+
+blest_status blest_device_graph_info_get(const blest_device_graph* h, blest_device_graph_info* i) {
+    return guarded([&] {
+        if (!h || !i) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        const DeviceGraph& g = h->g;
+        i->n = g.n;
+        i->rsets = g.R;
+        i->vsets = g.V;
+        i->grid_blocks = g.gridBlocks;
+        i->block_size = g.blockSize;
+        i->cus = uint32_t(g.cus);
+        i->symmetric = g.symmetric;
+        i->has_csr = g.hasCsr;
+        i->slices = g.S;
+        i->nnz = g.nnz;
+        i->device_bytes = g.deviceBytes;
+    });
+}
+
+blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, blest_query** out) {
+    return guarded([&] {
+        if (!dg || !out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        if (max_sources == 0) max_sources = 1;
+        DeviceGraph& g = dg->g;
+        BLEST_HIP(hipSetDevice(g.device));
+        auto* h = new blest_query();
+        Query& q = h->q;
+        try {
+            q.g = &g;
+            q.maxSources = max_sources;
+            BLEST_HIP(hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking));
+            BLEST_HIP(hipEventCreate(&q.ev0));
+            BLEST_HIP(hipEventCreate(&q.ev1));
+            BLEST_HIP(hipEventCreateWithFlags(&q.evChunk, hipEventDisableTiming));
+            BLEST_HIP(hipHostMalloc(reinterpret_cast<void**>(&q.hostDone), sizeof(unsigned), hipHostMallocDefault));
+            q.ctrl = q.alloc<dev::Ctrl>(1);
+            BLEST_HIP(hipMemset(q.ctrl, 0, sizeof(dev::Ctrl)));
+            q.levels = q.alloc<int32_t>(size_t(g.n) * max_sources);
+            q.visited = q.alloc<unsigned long long>(g.W);
+            q.front0 = q.alloc<unsigned long long>(g.W);
+            q.front1 = q.alloc<unsigned long long>(g.W);
+            q.next = q.alloc<unsigned long long>(g.W);
+            q.list0 = q.alloc<uint32_t>(g.V);
+            q.list1 = q.alloc<uint32_t>(g.V);
+            q.segCount = q.alloc<uint32_t>(3 * size_t(g.segs));
+            q.segNew = q.alloc<uint32_t>(3 * size_t(g.segs));
+            q.segEdges = q.alloc<unsigned long long>(3 * size_t(g.segs));
+            if (g.hasPerm) q.levelsOut = q.alloc<int32_t>(size_t(g.n) * max_sources);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void blest_query_free(blest_query* q) { delete q; }
+
+
+}  // extern "C"

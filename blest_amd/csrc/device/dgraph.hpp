@@ -10,6 +10,7 @@
 
 #include "../capi_internal.hpp"
 #include "dev_common.hpp"
+#include "segments.hpp"
 
 namespace blest {
 
@@ -46,7 +47,10 @@ struct GraphView {
 struct DeviceGraph {
     int device = 0;
     int cus = 0;
-    uint32_t blockSize = 256, blocksPerCu = 8, gridBlocks = 0;
+    uint32_t blockSize = 256, blocksPerCu = 4, gridBlocks = 0;
+    uint32_t segs = 1, wps = 1;    // producer segments and bitmap words per segment
+    uint32_t* segBase = nullptr;   // segs + 1 first list slots
+    dev::SegView seg_view() const;
     blest_device_opts opts{};
     uint32_t n = 0, R = 0, V = 0, W = 0;
     uint64_t S = 0, Q = 0, nnz = 0;
@@ -99,6 +103,9 @@ struct Query {
     int32_t* levels = nullptr;  // n * maxSources
     unsigned long long *visited = nullptr, *front0 = nullptr, *front1 = nullptr, *next = nullptr;
     uint32_t *list0 = nullptr, *list1 = nullptr;
+    uint32_t *segCount = nullptr, *segNew = nullptr;  // [3][segs]
+    unsigned long long* segEdges = nullptr;            // [3][segs]
+    dev::SegCounters seg_counters() const;
     int32_t* levelsOut = nullptr;  // original-order copy (when the graph carries a permutation)
     uint32_t* labels = nullptr;    // CC
     uint32_t* labelsAux = nullptr;
@@ -124,7 +131,8 @@ struct Query {
 // Level-loop driver (bfs.hip): enqueues level(L) for L = 1, 2, ... in chunks and polls
 // the device's done flag after each chunk.
 dim3 level_grid(const DeviceGraph& g);
-void run_levels(Query& q, const std::function<void(unsigned)>& level);
+void run_levels(Query& q, const std::function<void(unsigned)>& level,
+                const std::function<void()>& tail = nullptr);
 uint32_t map_source(const DeviceGraph& g, uint32_t src);
 void bfs_launch(Query& q, uint32_t srcReordered);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "dev_common.hpp"
+#include "segments.hpp"
 
 namespace blest {
 namespace dev {
@@ -16,14 +17,18 @@ struct DirectionParams {
     double totalItems;              // vertices (or vertex-root pairs)
 };
 
-// Returns false (and, in block 0, records the final totals and raises ctrl->done) when
-// the previous level discovered nothing.  Must be called by every thread of the block.
-__device__ __forceinline__ bool level_begin(Ctrl* c, unsigned L, const DirectionParams& p, bool& bottomUp) {
+// Reads what level L-1 produced (per-segment counters of slot L % 3), decides level
+// L's direction and returns false when the previous level discovered nothing (then
+// block 0 records the final totals and raises ctrl->done).  Every thread of the block
+// must call it.
+__device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint32_t segs, unsigned L,
+                                            const DirectionParams& p, bool& bottomUp) {
+    __shared__ unsigned long long s_tmp[16];
     if (ld_relaxed(&c->done)) return false;
     const LevelState in = c->state[(L - 1) & 1];
     const unsigned sIn = L % 3;
-    const unsigned nf = ld_relaxed(&c->newCount[sIn]);
-    const unsigned long long mf = ld_relaxed64(&c->frontEdges[sIn]);
+    const unsigned long long nf = block_sum_row_u32(sc.news + size_t(sIn) * segs, segs, s_tmp);
+    const unsigned long long mf = block_sum_row_u64(sc.edges + size_t(sIn) * segs, segs, s_tmp);
     __syncthreads();  // every thread has read state / counters before block 0 writes
     if (nf == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -52,29 +57,20 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, unsigned L, const Direction
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         LevelState out;
         out.dir = bottomUp ? 1u : 0u;
-        out.prevNf = nf;
-        out.visited = in.visited + nf;
+        out.prevNf = unsigned(min(nf, 0xFFFFFFFFull));
+        out.visited = unsigned(min(in.visited + nf, 0xFFFFFFFFull));
         out.td = in.td + (bottomUp ? 0u : 1u);
         out.bu = in.bu + (bottomUp ? 1u : 0u);
         out.pad = 0;
         out.cumEdges = cum;
         out.buMask = in.buMask | ((bottomUp && L <= 64) ? (1ull << (L - 1)) : 0ull);
         c->state[L & 1] = out;
-        const unsigned sRst = (L + 2) % 3;
-        c->listCount[sRst] = 0;
-        c->newCount[sRst] = 0;
-        c->frontEdges[sRst] = 0;
     }
     return true;
 }
 
 __device__ __forceinline__ void level_state_reset(Ctrl* c) {
     c->done = 0;
-    for (int s = 0; s < 3; ++s) {
-        c->listCount[s] = 0;
-        c->newCount[s] = 0;
-        c->frontEdges[s] = 0;
-    }
     c->state[0] = LevelState{0, 0, 0, 0, 0, 0, 0ull, 0ull};
     c->state[1] = c->state[0];
 }

@@ -4,15 +4,17 @@
 //
 // Every vertex carries NW 64-bit words (NW = 1: 64 roots, NW = 2: 128 roots): bit j of
 // visited[v] / frontier[v] says whether root j has reached / just reached v.  Level L is
-// two stream-ordered kernels, like the single-source traversal:
+// two stream-ordered kernels, like the single-source traversal, with the same
+// segmented (atomic-free) frontier lists and counters:
 //   expand  top-down: a slice (row i, mask m) of rset r contributes OR_{b in m} F[8r+b]
 //             to row i: the boolean product of the slice with the 8 x 64*NW frontier
 //             block of its rset.  Roots not yet at i are OR-ed into next[i].
-//           bottom-up: vertex per thread: the roots that have not reached v pull the
-//             OR of their in-neighbours' frontier words, stopping once all are found.
-//   update  top-down: vertex per thread: new = next[v]; visited |= new; levels of the
-//             new (root, vertex) pairs; the vsets of rsets holding a frontier vertex of
-//             any root form the next list.  bottom-up: the list from the new frontier.
+//           bottom-up: workgroup per segment, lane per vertex: the roots that have not
+//             reached v pull the OR of their in-neighbours' frontier words, stopping
+//             once all are found.
+//   update  top-down: lane per vertex: new = next[v]; visited |= new; levels of the new
+//             (root, vertex) pairs; both directions list the vsets of rsets holding a
+//             frontier vertex of any root.
 // Levels are written root-major: levels[j * n + v].
 #include "dgraph.hpp"
 #include "level_common.hpp"
@@ -31,6 +33,8 @@ struct MbfsArgs {
     const uint32_t* srcs;
     uint32_t k;
     Ctrl* ctrl;
+    SegView sv;
+    SegCounters sc;
     int direction;
     double alpha, beta;
 };
@@ -64,28 +68,6 @@ struct Bits {
     }
 };
 
-// Wave-cooperative: lane 0 carries the wave's 64-vertex activity word `bits` (vertices
-// [64 word, 64 word + 64)); the vsets of every rset with an active vertex are appended.
-__device__ __forceinline__ void mbfs_enqueue(const GraphView& g, uint32_t word, unsigned long long bits,
-                                             uint32_t* list, unsigned* counter) {
-    bits = __shfl(bits, 0, kWave);
-    if (!bits) return;
-    const unsigned l = lane_id();
-    unsigned mine = 0;
-    if (l < 8 && ((bits >> (8 * l)) & 0xFFull)) mine = g.rsetUnit[word * 8 + l + 1] - g.rsetUnit[word * 8 + l];
-    const unsigned incl = wave_inclusive_scan(mine);
-    const unsigned total = __shfl(incl, kWave - 1, kWave);
-    if (total == 0) return;
-    unsigned base = 0;
-    if (l == 0) base = atomicAdd(counter, total);
-    base = __shfl(base, 0, kWave);
-    if (mine) {
-        const uint32_t r = word * 8 + l;
-        const unsigned off = base + incl - mine;
-        for (uint32_t u = 0; u < mine; ++u) list[off + u] = g.rsetUnit[r] + u;
-    }
-}
-
 template <int NW>
 __device__ __forceinline__ Bits<NW> root_mask(uint32_t k) {
     Bits<NW> m;
@@ -97,18 +79,57 @@ __device__ __forceinline__ Bits<NW> root_mask(uint32_t k) {
     return m;
 }
 
+// Activity word of vertex word w: bit l = some root has vertex 64w+l in frontier f.
 template <int NW>
-__global__ void __launch_bounds__(256) mbfs_init_kernel(GraphView g, MbfsArgs a) {
+__device__ __forceinline__ unsigned long long activity(const GraphView& g, const unsigned long long* f, uint32_t w) {
+    const uint32_t v = w * 64 + lane_id();
+    Bits<NW> b;
+    b.zero();
+    if (v < g.n) b.load(f, v);
+    return __ballot(b.any());
+}
+
+// Writes the vsets of the rsets active in frontier f over this wave's words
+// [first, last) at this workgroup's list range, placed by one block scan over the
+// waves' counts.  Returns the segment total.  Must be called by the whole block.
+template <int NW>
+__device__ __forceinline__ unsigned write_segment_list(const GraphView& g, const SegView& sv, uint32_t first,
+                                                       uint32_t last, unsigned cnt, const unsigned long long* f,
+                                                       uint32_t* list, unsigned* tmp) {
+    unsigned total;
+    uint32_t* out = list + sv.segBase[blockIdx.x] + wave_offset_in_block(cnt, tmp, total);
+    for (uint32_t w = first; w < last; ++w) out += wave_write_word_vsets(g.rsetUnit, w, activity<NW>(g, f, w), out);
+    return total;
+}
+
+// The vset list of a frontier held as per-vertex words, for this workgroup's segment.
+template <int NW>
+__device__ __forceinline__ unsigned segment_list(const GraphView& g, const SegView& sv,
+                                                 const unsigned long long* f, uint32_t* list, unsigned* tmp) {
+    const uint32_t ws = blockIdx.x * sv.wps, we = min(g.W, ws + sv.wps);
+    uint32_t first, last;
+    wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
+    unsigned cnt = 0;
+    for (uint32_t w = first; w < last; ++w) cnt += wave_word_vsets(g.rsetUnit, w, activity<NW>(g, f, w));
+    return write_segment_list<NW>(g, sv, first, last, cnt, f, list, tmp);
+}
+
+template <int NW>
+__global__ void __launch_bounds__(kProducerBlock) mbfs_init_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     for (size_t i = tid; i < size_t(g.n) * NW; i += nthreads) {
         a.visited[i] = 0ull;
         a.front1[i] = 0ull;
         a.next[i] = 0ull;
     }
+    for (uint32_t s = tid; s < a.sv.segs; s += nthreads) {
+        a.sc.news[a.sv.segs + s] = 0;
+        a.sc.edges[a.sv.segs + s] = 0;
+    }
     if (tid == 0) level_state_reset(a.ctrl);
 }
 
-// Roots set their bits and level 0; counters of the first frontier.
+// Roots set their bits and level 0, and count into their segment's slot-1 counters.
 template <int NW>
 __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
     const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,34 +139,25 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
     atomicOr(&a.visited[size_t(v) * NW + j / 64], bit);
     atomicOr(&a.front1[size_t(v) * NW + j / 64], bit);
     a.levels[size_t(j) * g.n + v] = 0;
-    atomicAdd(&a.ctrl->newCount[1], 1u);
-    if (g.csrPtr) atomicAdd(&a.ctrl->frontEdges[1], (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]));
-}
-
-// The vset list of a frontier given as per-vertex words (first level; after bottom-up).
-template <int NW>
-__device__ __forceinline__ void mbfs_build_list(const GraphView& g, const unsigned long long* f, uint32_t* list,
-                                                unsigned* counter) {
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
-    for (uint32_t v0 = wave * kWave; v0 < g.W * 64; v0 += nwaves * kWave) {
-        const uint32_t v = v0 + lane_id();
-        Bits<NW> b;
-        b.zero();
-        if (v < g.n) b.load(f, v);
-        mbfs_enqueue(g, v0 / 64, __ballot(b.any()), list, counter);
-    }
+    const uint32_t seg = (v >> 6) / a.sv.wps;
+    atomicAdd(&a.sc.news[a.sv.segs + seg], 1u);
+    if (g.csrPtr) atomicAdd(&a.sc.edges[a.sv.segs + seg], (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]));
 }
 
 template <int NW>
-__global__ void __launch_bounds__(256) mbfs_first_list_kernel(GraphView g, MbfsArgs a) {
-    mbfs_build_list<NW>(g, a.front1, a.list1, &a.ctrl->listCount[1]);
+__global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphView g, MbfsArgs a) {
+    __shared__ unsigned tmp[16];
+    const unsigned cnt = segment_list<NW>(g, a.sv, a.front1, a.list1, tmp);
+    if (threadIdx.x == 0) a.sc.count[a.sv.segs + blockIdx.x] = cnt;
 }
 
 template <int NW>
-__global__ void __launch_bounds__(256) mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned L) {
-    __shared__ BlockCounts sc;
+__global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned L) {
+    __shared__ unsigned pre[kMaxSegs];
+    __shared__ unsigned tmp[16];
+    __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
+    const uint32_t segs = a.sv.segs;
     DirectionParams dp;
     dp.direction = a.direction;
     dp.canBottomUp = g.csrPtr != nullptr;
@@ -154,34 +166,35 @@ __global__ void __launch_bounds__(256) mbfs_expand_kernel(GraphView g, MbfsArgs 
     dp.totalEdges = (unsigned long long)a.k * g.nnz;
     dp.totalItems = double(g.n) * a.k;
     bool bottomUp = false;
-    if (!level_begin(c, L, dp, bottomUp)) return;
+    if (!level_begin(c, a.sc, segs, L, dp, bottomUp)) return;
     const unsigned lane = lane_id();
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
     const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    const unsigned sIn = L % 3, sOut = (L + 1) % 3;
     if (!bottomUp) {
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
-        const unsigned cnt = ld_relaxed(&c->listCount[L % 3]);
-        for (unsigned base = wave * kWave; base < cnt; base += nwaves * kWave) {
+        const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
+        const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+        const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
+        for (unsigned base = wave * kWave; base < total; base += nwaves * kWave) {
             const unsigned e = base + lane;
             uint32_t q0 = 0, qc = 0, r = 0;
-            if (e < cnt) {
-                const uint32_t vs = lin[e];
+            if (e < total) {
+                const uint32_t vs = list_entry(lin, a.sv, pre, e);
                 q0 = g.unitQuad[vs];
                 qc = g.unitQuad[vs + 1] - q0;
                 r = g.unitRset[vs];
             }
             const unsigned incl = wave_inclusive_scan(qc);
             const unsigned excl = incl - qc;
-            const unsigned total = __shfl(incl, kWave - 1, kWave);
-            for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+            const unsigned tq = __shfl(incl, kWave - 1, kWave);
+            for (unsigned t0 = 0; t0 < tq; t0 += kWave) {
                 const unsigned t = t0 + lane;
-                const unsigned owner = wave_owner(excl, min(t, total - 1));
+                const unsigned owner = wave_owner(excl, min(t, tq - 1));
                 const uint32_t oq0 = __shfl(q0, owner, kWave);
                 const unsigned oex = __shfl(excl, owner, kWave);
                 const uint32_t orr = __shfl(r, owner, kWave);
-                if (t < total) {
+                if (t < tq) {
                     const uint32_t q = oq0 + (t - oex);
                     const uint4 r4 = g.rows[q];
                     const uint32_t m = g.masks[q];
@@ -208,8 +221,9 @@ __global__ void __launch_bounds__(256) mbfs_expand_kernel(GraphView g, MbfsArgs 
                         const uint32_t row = rowv[s];
 #pragma unroll
                         for (int i = 0; i < NW; ++i) {
-                            const unsigned long long nv = acc.w[i] & ~a.visited[size_t(row) * NW + i];
-                            if (nv) atomicOr(&a.next[size_t(row) * NW + i], nv);
+                            const size_t at = size_t(row) * NW + i;
+                            const unsigned long long nv = acc.w[i] & ~(a.visited[at] | a.next[at]);
+                            if (nv) atomicOr(&a.next[at], nv);
                         }
                     }
                 }
@@ -217,76 +231,83 @@ __global__ void __launch_bounds__(256) mbfs_expand_kernel(GraphView g, MbfsArgs 
         }
         return;
     }
-    // bottom-up: vertex per thread
-    block_counts_init(&sc);
+    // bottom-up over this workgroup's segment: wave per word, lane per vertex
     const Bits<NW> rootMask = root_mask<NW>(a.k);
-    for (uint32_t v0 = wave * kWave; v0 < g.W * 64; v0 += nwaves * kWave) {
-        const uint32_t v = v0 + lane;
-        Bits<NW> nw;
+    const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
+    const unsigned wib = threadIdx.x / kWave, wpb = blockDim.x / kWave;
+    unsigned long long news = 0, edges = 0;
+    for (uint32_t w = ws + wib; w < we; w += wpb) {
+        const uint32_t v = w * 64 + lane;
+        if (v >= g.n) continue;
+        Bits<NW> vis, U, nw;
+        vis.load(a.visited, v);
         nw.zero();
-        unsigned long long edges = 0;
-        if (v < g.n) {
-            Bits<NW> vis, U;
-            vis.load(a.visited, v);
-            bool need = false;
+        bool need = false;
 #pragma unroll
-            for (int i = 0; i < NW; ++i) {
-                U.w[i] = ~vis.w[i] & rootMask.w[i];
-                need |= U.w[i] != 0;
-            }
-            if (need) {
-                Bits<NW> acc;
-                acc.zero();
-                const uint32_t p0 = g.csrPtr[v], p1 = g.csrPtr[v + 1];
-                for (uint32_t p = p0; p < p1; ++p) {
-                    const uint32_t u = g.csrIdx[p];
-                    bool all = true;
-#pragma unroll
-                    for (int i = 0; i < NW; ++i) {
-                        acc.w[i] |= fin[size_t(u) * NW + i];
-                        all &= (acc.w[i] & U.w[i]) == U.w[i];
-                    }
-                    if (all) break;
-                }
-#pragma unroll
-                for (int i = 0; i < NW; ++i) nw.w[i] = acc.w[i] & U.w[i];
-                if (nw.any()) {
-#pragma unroll
-                    for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
-                    vis.store(a.visited, v);
-                    for (int i = 0; i < NW; ++i)
-                        for (unsigned long long b = nw.w[i]; b; b &= b - 1)
-                            a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
-                    edges = (unsigned long long)(p1 - p0) * nw.popc();
-                }
-            }
-            nw.store(fout, v);
+        for (int i = 0; i < NW; ++i) {
+            U.w[i] = ~vis.w[i] & rootMask.w[i];
+            need |= U.w[i] != 0;
         }
-        block_counts_add(&sc, wave_sum_u32(nw.popc()), wave_sum_u64(edges));
+        if (need) {
+            Bits<NW> acc;
+            acc.zero();
+            const uint32_t p0 = g.csrPtr[v], p1 = g.csrPtr[v + 1];
+            for (uint32_t p = p0; p < p1; ++p) {
+                const uint32_t u = g.csrIdx[p];
+                bool all = true;
+#pragma unroll
+                for (int i = 0; i < NW; ++i) {
+                    acc.w[i] |= fin[size_t(u) * NW + i];
+                    all &= (acc.w[i] & U.w[i]) == U.w[i];
+                }
+                if (all) break;
+            }
+#pragma unroll
+            for (int i = 0; i < NW; ++i) nw.w[i] = acc.w[i] & U.w[i];
+            if (nw.any()) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
+                vis.store(a.visited, v);
+                for (int i = 0; i < NW; ++i)
+                    for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                        a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                news += nw.popc();
+                edges += (unsigned long long)(p1 - p0) * nw.popc();
+            }
+        }
+        nw.store(fout, v);
     }
-    const unsigned sOut = (L + 1) % 3;
-    block_counts_flush(&sc, &c->newCount[sOut], &c->frontEdges[sOut]);
+    news = block_sum_u64(news, tmp64);
+    edges = block_sum_u64(edges, tmp64);
+    if (threadIdx.x == 0) {
+        a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(min(news, 0xFFFFFFFFull));
+        a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;
+    }
 }
 
 template <int NW>
-__global__ void __launch_bounds__(256) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned L) {
-    __shared__ BlockCounts sc;
+__global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned L) {
+    __shared__ unsigned tmp[16];
+    __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
     const bool bottomUp = c->state[L & 1].dir != 0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     uint32_t* lout = (L & 1) ? a.list0 : a.list1;
+    const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
     if (bottomUp) {
-        mbfs_build_list<NW>(g, fout, lout, &c->listCount[sOut]);
+        const unsigned cnt = segment_list<NW>(g, a.sv, fout, lout, tmp);
+        if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = cnt;
         return;
     }
-    block_counts_init(&sc);
-    const unsigned lane = lane_id();
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned wave = tid / kWave, nwaves = (gridDim.x * blockDim.x) / kWave;
-    for (uint32_t v0 = wave * kWave; v0 < g.W * 64; v0 += nwaves * kWave) {
-        const uint32_t v = v0 + lane;
+    const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
+    uint32_t first, last;
+    wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
+    unsigned cnt = 0;
+    unsigned long long news = 0, edges = 0;
+    for (uint32_t w = first; w < last; ++w) {
+        const uint32_t v = w * 64 + lane_id();
         Bits<NW> nw;
         nw.zero();
         if (v < g.n) {
@@ -299,26 +320,30 @@ __global__ void __launch_bounds__(256) mbfs_update_kernel(GraphView g, MbfsArgs 
 #pragma unroll
                 for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
                 vis.store(a.visited, v);
+                const unsigned pc = nw.popc();
+                for (int i = 0; i < NW; ++i)
+                    for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                        a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                news += pc;
+                if (g.csrPtr) edges += (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]) * pc;
             }
             nw.store(fout, v);
         }
-        const unsigned cntNew = nw.popc();
-        unsigned long long edges = 0;
-        if (cntNew) {
-            for (int i = 0; i < NW; ++i)
-                for (unsigned long long b = nw.w[i]; b; b &= b - 1)
-                    a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
-            if (g.csrPtr) edges = (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]) * cntNew;
-        }
-        mbfs_enqueue(g, v0 / 64, __ballot(cntNew != 0), lout, &c->listCount[sOut]);
-        block_counts_add(&sc, wave_sum_u32(cntNew), wave_sum_u64(edges));
+        cnt += wave_word_vsets(g.rsetUnit, w, __ballot(nw.any()));
     }
-    block_counts_flush(&sc, &c->newCount[sOut], &c->frontEdges[sOut]);
+    const unsigned running = write_segment_list<NW>(g, a.sv, first, last, cnt, fout, lout, tmp);
+    news = block_sum_u64(news, tmp64);
+    edges = block_sum_u64(edges, tmp64);
+    if (threadIdx.x == 0) {
+        a.sc.count[size_t(sOut) * segs + blockIdx.x] = running;
+        a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(min(news, 0xFFFFFFFFull));
+        a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;
+    }
 }
 
 // Unreached (root, vertex) pairs get -1; root-major rows, so stores are coalesced.
 template <int NW>
-__global__ void __launch_bounds__(256) mbfs_final_kernel(GraphView g, MbfsArgs a) {
+__global__ void __launch_bounds__(kProducerBlock) mbfs_final_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     for (uint32_t v = tid; v < g.n; v += nthreads) {
         Bits<NW> vis;
@@ -353,15 +378,17 @@ template <int NW>
 static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     DeviceGraph& g = *q.g;
     const GraphView gv = g.view();
-    const dim3 grid = level_grid(g), block(256);
+    const dim3 grid = level_grid(g), block(dev::kProducerBlock);
     hipLaunchKernelGGL(dev::mbfs_init_kernel<NW>, grid, block, 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_seed_kernel<NW>, dim3(2), dim3(64), 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_first_list_kernel<NW>, grid, block, 0, q.stream, gv, a);
-    run_levels(q, [&](unsigned L) {
-        hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
-        hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
-    });
-    hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, grid, block, 0, q.stream, gv, a);
+    run_levels(
+        q,
+        [&](unsigned L) {
+            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
+        },
+        [&] { hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, grid, block, 0, q.stream, gv, a); });
     BLEST_HIP(hipGetLastError());
 }
 
@@ -387,6 +414,8 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.srcs = q.srcs;
     a.k = k;
     a.ctrl = q.ctrl;
+    a.sv = g.seg_view();
+    a.sc = q.seg_counters();
     a.direction = g.hasCsr ? g.opts.direction : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;

@@ -1,0 +1,171 @@
+// Contention-free frontier lists and counters.
+//
+// Same-address atomics from many workgroups serialise at the device's coherence point
+// (~80 ns each on MI355X), so producers never share a counter.  The bitmap words are cut
+// into `segs` contiguous segments, one per producer workgroup.  Because the vsets of an
+// rset are numbered contiguously and rsets ascend with the words, the vsets a segment
+// can emit lie in the fixed list range [segBase[s], segBase[s+1]); the producer writes
+// its vsets compactly from segBase[s] (block-wide scan, no atomics) and stores the
+// count in segCount[s].  Per-segment vertex and edge counts are stored the same way.
+// Consumers read the segs counts and take a prefix in LDS.
+#pragma once
+
+#include "dev_common.hpp"
+
+namespace blest {
+namespace dev {
+
+constexpr int kMaxSegs = 2048;
+constexpr int kProducerBlock = 512;
+
+struct SegView {
+    uint32_t segs;            // producer workgroups
+    uint32_t wps;             // bitmap words per segment
+    const uint32_t* segBase;  // segs + 1: first list slot of each segment
+};
+
+// Per-slot per-segment counters (slot = level % 3).
+struct SegCounters {
+    uint32_t* count;          // [3][segs] list entries
+    uint32_t* news;           // [3][segs] vertices (or vertex-root pairs) discovered
+    unsigned long long* edges;  // [3][segs] their degree sum
+};
+
+// Exclusive scan over the block (blockDim.x <= 1024, multiple of 64); returns the
+// thread's exclusive prefix and the block total.  `tmp` holds >= 16 entries.
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned x, unsigned* tmp, unsigned& total) {
+    const unsigned lane = lane_id(), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const unsigned incl = wave_inclusive_scan(x);
+    __syncthreads();
+    if (lane == kWave - 1) tmp[wave] = incl;
+    __syncthreads();
+    unsigned before = 0, all = 0;
+    for (unsigned i = 0; i < nw; ++i) {
+        const unsigned t = tmp[i];
+        before += (i < wave) ? t : 0u;
+        all += t;
+    }
+    total = all;
+    return before + incl - x;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x, unsigned long long* tmp) {
+    const unsigned lane = lane_id(), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    x = wave_sum_u64(x);
+    __syncthreads();
+    if (lane == 0) tmp[wave] = x;
+    __syncthreads();
+    unsigned long long s = 0;
+    for (unsigned i = 0; i < nw; ++i) s += tmp[i];
+    return s;
+}
+
+// Sum of a [segs] counter row, computed by the whole block (every thread gets it).
+__device__ __forceinline__ unsigned long long block_sum_row_u32(const uint32_t* row, uint32_t segs,
+                                                                unsigned long long* tmp) {
+    unsigned long long s = 0;
+    for (uint32_t i = threadIdx.x; i < segs; i += blockDim.x) s += row[i];
+    return block_sum_u64(s, tmp);
+}
+__device__ __forceinline__ unsigned long long block_sum_row_u64(const unsigned long long* row, uint32_t segs,
+                                                                unsigned long long* tmp) {
+    unsigned long long s = 0;
+    for (uint32_t i = threadIdx.x; i < segs; i += blockDim.x) s += row[i];
+    return block_sum_u64(s, tmp);
+}
+
+// Loads a [segs] count row into LDS as an inclusive prefix (pre[i] = sum of counts
+// 0..i); returns the total.  pre holds kMaxSegs entries.
+__device__ __forceinline__ unsigned load_prefix(const uint32_t* row, uint32_t segs, unsigned* pre, unsigned* tmp) {
+    unsigned carry = 0;
+    for (uint32_t base = 0; base < segs; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const unsigned x = i < segs ? row[i] : 0u;
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(x, tmp, tot);
+        if (i < segs) pre[i] = carry + ex + x;
+        carry += tot;
+        __syncthreads();
+    }
+    __syncthreads();
+    return carry;
+}
+
+// Segment holding global entry index e (pre = inclusive prefix, e < total).
+__device__ __forceinline__ uint32_t find_segment(const unsigned* pre, uint32_t segs, unsigned e) {
+    uint32_t lo = 0, hi = segs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] > e) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// List entry with global index e.
+__device__ __forceinline__ uint32_t list_entry(const uint32_t* list, const SegView& sv, const unsigned* pre,
+                                               unsigned e) {
+    const uint32_t s = find_segment(pre, sv.segs, e);
+    const unsigned before = s ? pre[s - 1] : 0u;
+    return list[sv.segBase[s] + (e - before)];
+}
+
+// Number of vsets of the rsets with a nonzero byte in word w's bits.
+__device__ __forceinline__ unsigned word_vsets(const uint32_t* rsetUnit, uint32_t w, unsigned long long bits) {
+    unsigned cnt = 0;
+    if (bits)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if ((bits >> (8 * b)) & 0xFFull) cnt += rsetUnit[w * 8 + b + 1] - rsetUnit[w * 8 + b];
+    return cnt;
+}
+
+__device__ __forceinline__ void write_word_vsets(const uint32_t* rsetUnit, uint32_t w, unsigned long long bits,
+                                                 uint32_t* out) {
+    if (!bits) return;
+    for (int b = 0; b < 8; ++b)
+        if ((bits >> (8 * b)) & 0xFFull)
+            for (uint32_t u = rsetUnit[w * 8 + b]; u < rsetUnit[w * 8 + b + 1]; ++u) *out++ = u;
+}
+
+// Word range [first, last) of wave `wib` when a segment [ws, we) is cut into
+// contiguous runs, one per wave of the workgroup.
+__device__ __forceinline__ void wave_words(uint32_t ws, uint32_t we, unsigned wib, unsigned wpb, uint32_t& first,
+                                           uint32_t& last) {
+    const uint32_t per = (we - ws + wpb - 1) / wpb;
+    first = min(we, ws + wib * per);
+    last = min(we, first + per);
+}
+
+// Wave-level (lane = one of the word's 8 rsets) count of the vsets of the rsets with a
+// frontier vertex in `bits` (every lane gets the total).
+__device__ __forceinline__ unsigned wave_word_vsets(const uint32_t* rsetUnit, uint32_t w, unsigned long long bits) {
+    const unsigned l = lane_id();
+    unsigned mine = 0;
+    if (l < 8 && ((bits >> (8 * l)) & 0xFFull)) mine = rsetUnit[w * 8 + l + 1] - rsetUnit[w * 8 + l];
+    return wave_sum_u32(mine);
+}
+
+// Writes those vsets at out[0 ..), lanes 0..7 one rset each; returns the count.
+__device__ __forceinline__ unsigned wave_write_word_vsets(const uint32_t* rsetUnit, uint32_t w,
+                                                          unsigned long long bits, uint32_t* out) {
+    const unsigned l = lane_id();
+    unsigned mine = 0, start = 0;
+    if (l < 8 && ((bits >> (8 * l)) & 0xFFull)) {
+        start = rsetUnit[w * 8 + l];
+        mine = rsetUnit[w * 8 + l + 1] - start;
+    }
+    const unsigned incl = wave_inclusive_scan(mine);
+    for (unsigned u = 0; u < mine; ++u) out[incl - mine + u] = start + u;
+    return __shfl(incl, kWave - 1, kWave);
+}
+
+// Offset of this wave's output inside the segment, from each wave's count (one block
+// scan per kernel).  Returns the segment total in `total`.
+__device__ __forceinline__ unsigned wave_offset_in_block(unsigned waveCount, unsigned* tmp, unsigned& total) {
+    const unsigned ex = block_exclusive_scan(lane_id() == 0 ? waveCount : 0u, tmp, total);
+    return __shfl(ex, 0, kWave);
+}
+
+}  // namespace dev
+}  // namespace blest
