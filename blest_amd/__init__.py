@@ -25,7 +25,7 @@ from . import _lib
 from ._lib import BlestError, check
 
 __all__ = ["CSC", "BVSS", "BitMatrix", "DeviceGraph", "BVSSBFSKernel", "BVSSCCKernel", "BVSSMBFSKernel",
-           "BFSResult", "CCResult", "MBFSResult", "Reorder", "BlestError", "device_opts", "GraphBatch"]
+           "BFSResult", "CCResult", "MBFSResult", "Reorder", "BlestError", "device_opts", "GraphBatch", "Server"]
 
 
 def _u32p(a: np.ndarray):
@@ -454,3 +454,52 @@ class GraphBatch:
             out.append(lv[off:off + s])
             off += s
         return out, ms.value / 1e3
+
+
+class Server:
+    """Request server with continuous batching over resident device graphs.
+
+    submit() queues a single-root BFS ("bfs") or a connected-components ("cc") request
+    and returns a ticket; wait(ticket) returns (levels or labels, result dict).
+    """
+
+    KINDS = {"bfs": 0, "cc": 1}
+
+    def __init__(self, graphs: list[DeviceGraph], workers: int = 4, max_batch: int = 64,
+                 hbm_budget: int = 0, capacity: int = 65536):
+        self._lib = _lib.load()
+        o = _lib.ServerOpts()
+        self._lib.blest_server_opts_default(C.byref(o))
+        o.workers, o.max_batch, o.hbm_budget, o.capacity = workers, max_batch, hbm_budget, capacity
+        arr = (C.c_void_p * len(graphs))(*[g._h.value for g in graphs])
+        h = C.c_void_p()
+        check(self._lib.blest_server_create(arr, len(graphs), C.byref(o), C.byref(h)))
+        self._h = h
+        self._graphs = list(graphs)
+        self._n = [g.n for g in graphs]
+        self._out: dict[int, np.ndarray] = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.blest_server_free(h)
+            self._h = C.c_void_p(0)
+
+    def submit(self, graph: int, kind: str = "bfs", root: int = 0) -> int:
+        k = self.KINDS[kind]
+        out = np.empty(self._n[graph], dtype=np.int32 if k == 0 else np.uint32)
+        t = C.c_uint64()
+        check(self._lib.blest_server_submit(self._h, graph, k, root, out.ctypes.data_as(C.c_void_p), C.byref(t)))
+        self._out[t.value] = out
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = 0xFFFFFFFF) -> tuple[np.ndarray, dict]:
+        r = _lib.RequestResult()
+        st = self._lib.blest_server_wait(self._h, ticket, timeout_ms, C.byref(r))
+        if st != 8:  # the ticket stays live on a timeout
+            out = self._out.pop(ticket, None)
+        check(st)
+        return out, {k: getattr(r, k) for k, _ in r._fields_}
+
+    def drain(self) -> None:
+        check(self._lib.blest_server_drain(self._h))

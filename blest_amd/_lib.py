@@ -48,7 +48,7 @@ class DeviceGraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("rsets", C.c_uint32), ("vsets", C.c_uint32), ("grid_blocks", C.c_uint32),
                 ("block_size", C.c_uint32), ("cus", C.c_uint32), ("symmetric", C.c_uint32),
                 ("has_csr", C.c_uint32), ("slices", C.c_uint64), ("nnz", C.c_uint64),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("device", C.c_int32)]
 
 
 class BfsStats(C.Structure):
@@ -63,6 +63,18 @@ class MbfsOpts(C.Structure):
 
 class CcStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("no_components", C.c_uint32), ("iterations", C.c_uint32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("workers", C.c_uint32), ("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("capacity", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("batch_size", C.c_uint32),
+                ("queued_ms", C.c_double), ("service_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("total_levels", C.c_uint32), ("no_visited", C.c_uint32),
+                ("edges_traversed_component", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -117,6 +129,14 @@ PROTOTYPES = {
     "blest_query_levels_device": (P, [P]),
     "blest_mbfs": (C.c_int, [P, U32P, C.c_uint32, I32P, C.POINTER(MbfsOpts), C.POINTER(BfsStats)]),
     "blest_cc": (C.c_int, [P, C.c_int, U32P, C.POINTER(CcStats)]),
+    "blest_device_memory_info": (C.c_int, [C.c_int, U64P, U64P]),
+    "blest_query_footprint": (C.c_int, [P, C.c_uint32, U64P]),
+    "blest_server_opts_default": (None, [C.POINTER(ServerOpts)]),
+    "blest_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(ServerOpts), PP]),
+    "blest_server_submit": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint32, P, U64P]),
+    "blest_server_wait": (C.c_int, [P, C.c_uint64, C.c_uint32, C.POINTER(RequestResult)]),
+    "blest_server_drain": (C.c_int, [P]),
+    "blest_server_free": (None, [P]),
     "blest_graph_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_int, PP]),
     "blest_graph_batch_free": (None, [P]),
     "blest_graph_batch_bfs": (C.c_int, [P, U32P, I32P, DP]),

@@ -17,6 +17,11 @@ blest_status fail(blest_status s, const std::string& msg);
 const CSC& csc_of(const blest_csc* h);
 const BVSS& bvss_of(const blest_bvss* h);
 
+// Multi-source BFS writing the levels of root j into rows[j] (n entries each); used by
+// the request server to hand each request its own row without a staging copy.
+blest_status mbfs_rows(blest_query* q, const uint32_t* srcs, uint32_t k, int32_t* const* rows,
+                       blest_bfs_stats* st);
+
 // Runs fn, converting exceptions into a status code (and the thread's last error).
 template <class F>
 blest_status guarded(F&& fn) {

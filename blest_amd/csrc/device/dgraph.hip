@@ -187,6 +187,7 @@ blest_status blest_device_graph_info_get(const blest_device_graph* h, blest_devi
         i->slices = g.S;
         i->nnz = g.nnz;
         i->device_bytes = g.deviceBytes;
+        i->device = g.device;
     });
 }
 
@@ -228,6 +229,32 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
 }
 
 void blest_query_free(blest_query* q) { delete q; }
+
+blest_status blest_device_memory_info(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        BLEST_HIP(hipSetDevice(device));
+        size_t f = 0, t = 0;
+        BLEST_HIP(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+    });
+}
+
+// Mirrors blest_query_create plus the lazily allocated multi-source and CC state.
+blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_sources, uint64_t* bytes) {
+    return guarded([&] {
+        if (!dg || !bytes) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        const DeviceGraph& g = dg->g;
+        if (max_sources == 0) max_sources = 1;
+        uint64_t b = sizeof(dev::Ctrl);
+        const uint64_t levels = 4ull * g.n * max_sources;
+        b += levels * (g.hasPerm ? 2 : 1);
+        b += 4ull * 8ull * g.W + 2ull * 4ull * g.V + 3ull * 16ull * g.segs;
+        if (max_sources > 1) b += 4ull * 8ull * g.n * (max_sources > 64 ? 2 : 1) + 4ull * 128;
+        b += 2ull * 4ull * g.n;  // CC labels
+        *bytes = b;
+    });
+}
 
 
 }  // extern "C"

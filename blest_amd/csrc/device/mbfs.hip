@@ -392,8 +392,9 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     BLEST_HIP(hipGetLastError());
 }
 
-static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLevels, const blest_mbfs_opts* o,
-                     blest_bfs_stats* st) {
+// hostLevels: k x n contiguous rows, or rows: k separate arrays of n (either may be null).
+static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLevels, int32_t* const* rows,
+                     const blest_mbfs_opts* o, blest_bfs_stats* st) {
     DeviceGraph& g = *q.g;
     if (k == 0 || k > 128) throw Error(BLEST_ERR_INVALID_ARGUMENT, "multi-source batches hold 1..128 roots");
     if (k > q.maxSources) throw Error(BLEST_ERR_INVALID_ARGUMENT, "more roots than the query's max_sources");
@@ -438,7 +439,7 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
         st->edges_traversed_component = c.visitedEdges / 2;
         st->bottom_up_mask = c.buMask;
     }
-    if (hostLevels) {
+    if (hostLevels || rows) {
         const int32_t* src = q.levels;
         if (g.hasPerm) {
             hipLaunchKernelGGL(dev::unpermute_rows_kernel, dim3(256, std::min<uint32_t>(k, 65535)), dim3(256), 0,
@@ -446,8 +447,13 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
             BLEST_HIP(hipGetLastError());
             src = q.levelsOut;
         }
-        BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * size_t(g.n) * k, hipMemcpyDeviceToHost,
-                                 q.stream));
+        if (hostLevels)
+            BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * size_t(g.n) * k, hipMemcpyDeviceToHost,
+                                     q.stream));
+        else
+            for (uint32_t j = 0; j < k; ++j)
+                BLEST_HIP(hipMemcpyAsync(rows[j], src + size_t(g.n) * j, sizeof(int32_t) * size_t(g.n),
+                                         hipMemcpyDeviceToHost, q.stream));
         BLEST_HIP(hipStreamSynchronize(q.stream));
     }
 }
@@ -463,6 +469,17 @@ extern "C" blest_status blest_mbfs(blest_query* qh, const uint32_t* srcs, uint32
         Query& q = qh->q;
         std::lock_guard<std::mutex> lk(q.mu);
         BLEST_HIP(hipSetDevice(q.g->device));
-        mbfs_run(q, srcs, k, levels, o, st);
+        mbfs_run(q, srcs, k, levels, nullptr, o, st);
+    });
+}
+
+blest_status blest::mbfs_rows(blest_query* qh, const uint32_t* srcs, uint32_t k, int32_t* const* rows,
+                              blest_bfs_stats* st) {
+    return guarded([&] {
+        if (!qh || !srcs || !rows) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        Query& q = qh->q;
+        std::lock_guard<std::mutex> lk(q.mu);
+        BLEST_HIP(hipSetDevice(q.g->device));
+        mbfs_run(q, srcs, k, nullptr, rows, nullptr, st);
     });
 }

@@ -195,6 +195,7 @@ void blest_device_graph_free(blest_device_graph* dg);
 typedef struct blest_device_graph_info {
     uint32_t n, rsets, vsets, grid_blocks, block_size, cus, symmetric, has_csr;
     uint64_t slices, nnz, device_bytes;
+    int32_t device;
 } blest_device_graph_info;
 blest_status blest_device_graph_info_get(const blest_device_graph* dg, blest_device_graph_info* i);
 
@@ -243,6 +244,53 @@ typedef struct blest_cc_stats {
     uint32_t iterations;
 } blest_cc_stats;
 blest_status blest_cc(blest_query* q, int algo, uint32_t* labels, blest_cc_stats* st);
+
+/* Free and total HBM of a device (hipMemGetInfo). */
+blest_status blest_device_memory_info(int device, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Device bytes one query needs for batches of up to max_sources roots. */
+blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_sources, uint64_t* bytes);
+
+/* ---------------------------------------------------------------- serving */
+/* A request server over resident device graphs.  Requests (single-root BFS or CC)
+ * queue as they arrive; `workers` host threads, each with its own HIP stream, take
+ * the oldest pending request and, for BFS, every other pending BFS request on the
+ * same graph (up to max_batch), and run them as one multi-source launch while the
+ * other workers' launches proceed on their streams (continuous batching).  A batch
+ * starts only when its query state fits the HBM byte budget.  No reference
+ * counterpart (the reference has no serving mode). */
+typedef struct blest_server blest_server;
+typedef struct blest_server_opts {
+    uint32_t workers;        /* concurrent batches (streams) */
+    uint32_t max_batch;      /* roots per multi-source launch (<= 128) */
+    uint64_t hbm_budget;     /* bytes for query state; 0 = 80% of free HBM at creation */
+    uint32_t capacity;       /* max queued requests (BLEST_ERR_BUSY beyond) */
+} blest_server_opts;
+void blest_server_opts_default(blest_server_opts* o);
+typedef enum blest_request_kind { BLEST_REQ_BFS = 0, BLEST_REQ_CC = 1 } blest_request_kind;
+typedef struct blest_request_result {
+    uint64_t ticket;
+    int32_t status;          /* blest_status of the request */
+    uint32_t batch_size;     /* requests that shared its launch */
+    double queued_ms;        /* submit -> launch */
+    double service_ms;       /* launch -> results copied */
+    double kernel_ms;
+    uint32_t total_levels, no_visited;
+    uint64_t edges_traversed_component;
+} blest_request_result;
+blest_status blest_server_create(blest_device_graph* const* graphs, uint32_t count, const blest_server_opts* o,
+                                 blest_server** out);
+/* Queue a request.  out: caller-owned host array of n entries of that graph
+ * (int32 levels for BFS, uint32 labels for CC) that must stay valid until the
+ * request completes. */
+blest_status blest_server_submit(blest_server* s, uint32_t graph, blest_request_kind kind, uint32_t root,
+                                 void* out, uint64_t* ticket);
+/* Block until the request completes (or timeout_ms passes: BLEST_ERR_TIMEOUT; 0xFFFFFFFF
+ * waits without limit), then release the ticket.  Returns the request's own status. */
+blest_status blest_server_wait(blest_server* s, uint64_t ticket, uint32_t timeout_ms,
+                               blest_request_result* res);
+/* Block until every submitted request has completed. */
+blest_status blest_server_drain(blest_server* s);
+void blest_server_free(blest_server* s);
 
 /* ------------------------------------------------------- batched graphs */
 /* Many small independent graphs, one root each, traversed by one persistent
