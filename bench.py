@@ -47,6 +47,155 @@ def harmonic(values):
     return float(len(v) / np.sum(1.0 / v))
 
 
+SERVE_RATES = (2000, 5000, 10000, 20000, 50000)
+
+
+def serve_bench(a) -> dict:
+    """Served traversal: resident R-MAT graphs (scales 18-25, edge factor 16) and two
+    perturbed grids (2048^2, 4096^2; keep 0.9, 1% shortcuts), natural order.  An open-loop
+    Poisson stream (80% single-root BFS, 20% CC; graph, root and arrival times seeded)
+    is submitted to the request server at each rate; results land in device buffers.
+    Latency = (submit - scheduled arrival) + queued + service, from the server's own
+    timestamps.  The same requests run one at a time through blest_bfs / blest_cc give
+    the comparison: their measured service times are replayed as a FIFO single server
+    under the same arrivals."""
+    import ctypes as C
+    import queue
+    import threading
+
+    import torch
+
+    import blest_amd as B
+    sys.setswitchinterval(1e-4)  # the submitter spins; let the waiter thread run promptly
+    lib = B._lib.load()
+    scales = [int(x) for x in a.serve_scales.split(",")] if a.serve_scales else list(range(18, 26))
+    sides = [int(x) for x in a.serve_grids.split(",")] if a.serve_grids else [2048, 4096]
+    graphs = []  # (label, DeviceGraph, n, roots)
+    t0 = time.time()
+    for s in scales:
+        g = B.CSC.rmat(s, 16, 0.57, 0.19, 0.19, seed=1000 + s, permute=True)
+        b = B.BVSS(8, 4).construct_from_csc(g, full_padding=not g.is_social_network())
+        graphs.append((f"rmat{s}", B.DeviceGraph(b, g, None), g.n, g.pick_roots(256, seed=s)))
+        log(f"[serve] rmat{s}: n={g.n} nnz={g.nnz} ({time.time() - t0:.0f}s)")
+    for side in sides:
+        g = B.CSC.grid(side, 0.9, 0.01, seed=2000 + side)
+        b = B.BVSS(8, 4).construct_from_csc(g, full_padding=not g.is_social_network())
+        graphs.append((f"grid{side}", B.DeviceGraph(b, g, None), g.n, g.pick_roots(256, seed=side)))
+        log(f"[serve] grid{side}: n={g.n} nnz={g.nnz} ({time.time() - t0:.0f}s)")
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(a.serve_seed)
+    N = a.serve_requests
+    gsel = rng.integers(len(graphs), size=N)
+    kinds = np.where(rng.random(N) < 0.8, 0, 1)
+    roots = np.array([int(graphs[gi][3][rng.integers(len(graphs[gi][3]))]) for gi in gsel], dtype=np.int64)
+    gaps = rng.exponential(1.0, size=N)  # unit-rate Poisson gaps, scaled per rate
+    per_graph_bufs = max(2, a.serve_buffers)
+    bufs = [[torch.empty(n, dtype=torch.int32, device=dev) for _ in range(per_graph_bufs)]
+            for _, _, n, _ in graphs]
+
+    # one-at-a-time reference: plain kernels, one query per graph, same requests
+    qs = [B._Query(dg, 1) for _, dg, _, _ in graphs]
+    bst, cst = B._lib.BfsStats(), B._lib.CcStats()
+    def plain(i):
+        gi, k, r = int(gsel[i]), int(kinds[i]), int(roots[i])
+        out = C.c_void_p(bufs[gi][0].data_ptr())
+        if k == 0:
+            B.check(lib.blest_bfs(qs[gi]._h, r, C.cast(out, B._lib.I32P), C.byref(bst)))
+            return bst.edges_traversed_component
+        B.check(lib.blest_cc(qs[gi]._h, 0, C.cast(out, B._lib.U32P), C.byref(cst)))
+        return 0
+    for i in range(min(N, 50)):
+        plain(i)
+    seq_service = np.empty(N)
+    mc = np.zeros(N)
+    for i in range(N):
+        t1 = time.perf_counter()
+        mc[i] = plain(i)
+        seq_service[i] = time.perf_counter() - t1
+    del qs
+    log(f"[serve] one-at-a-time: {N} requests in {seq_service.sum():.2f}s")
+
+    srv = B.Server([dg for _, dg, _, _ in graphs], workers=a.serve_workers, max_batch=a.serve_batch,
+                   capacity=per_graph_bufs * len(graphs))
+    free = [queue.SimpleQueue() for _ in graphs]
+    for gi, bl in enumerate(bufs):
+        for j in range(len(bl)):
+            free[gi].put(j)
+
+    def run_rate(rate, count):
+        arrivals = np.cumsum(gaps[:count]) / rate
+        lat = np.empty(count)
+        done_at = np.empty(count)
+        batch = np.empty(count)
+        pending = queue.SimpleQueue()
+
+        def waiter():
+            res = B._lib.RequestResult()
+            for _ in range(count):
+                i, t, gi, j, tsub = pending.get()
+                B.check(lib.blest_server_wait(srv._h, t, 0xFFFFFFFF, C.byref(res)))
+                free[gi].put(j)
+                svc = (res.queued_ms + res.service_ms) * 1e-3
+                lat[i] = (tsub - arrivals[i]) + svc
+                done_at[i] = tsub + svc
+                batch[i] = res.batch_size
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t = C.c_uint64()
+        start = time.perf_counter()
+        for i in range(count):
+            target = start + arrivals[i]
+            while True:
+                now = time.perf_counter()
+                if now >= target:
+                    break
+                if target - now > 2e-3:
+                    time.sleep(target - now - 1e-3)
+            gi = int(gsel[i])
+            j = free[gi].get()
+            tsub = time.perf_counter() - start
+            B.check(lib.blest_server_submit(srv._h, gi, int(kinds[i]), int(roots[i]),
+                                            C.c_void_p(bufs[gi][j].data_ptr()), C.byref(t)))
+            pending.put((i, t.value, gi, j, tsub))
+        th.join()
+        return lat, float(done_at.max()), batch
+
+    run_rate(SERVE_RATES[0], min(N, 200))  # warm the server's query pool
+    rows = []
+    for rate in (a.serve_rates or SERVE_RATES):
+        lat, wall, batch = run_rate(rate, N)
+        # FIFO single server with the measured one-at-a-time service times
+        arr = np.cumsum(gaps[:N]) / rate
+        fin = np.empty(N)
+        tcur = 0.0
+        for i in range(N):
+            tcur = max(tcur, arr[i]) + seq_service[i]
+            fin[i] = tcur
+        seq_lat = fin - arr
+        row = {"rate": rate, "requests": N,
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "throughput_rps": round(N / wall, 1), "bfs_gteps": round(float(mc.sum()) / wall / 1e9, 3),
+               "mean_batch": round(float(batch.mean()), 2),
+               "one_at_a_time": {"p50_ms": round(float(np.percentile(seq_lat, 50)) * 1e3, 3),
+                                 "p99_ms": round(float(np.percentile(seq_lat, 99)) * 1e3, 3),
+                                 "throughput_rps": round(N / float(fin[-1]), 1),
+                                 "bfs_gteps": round(float(mc.sum()) / float(fin[-1]) / 1e9, 3)}}
+        rows.append(row)
+        log(f"[serve] {json.dumps(row)}")
+    del srv
+    head = next((r for r in rows if r["rate"] == 10000), rows[-1])
+    return {"metric": f"served p99 latency at {head['rate']} req/s (80% BFS / 20% CC)", "value": head["p99_ms"],
+            "unit": "ms", "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": None,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic",
+            "config": {"workload": "serve", "graphs": [x[0] for x in graphs], "reorder": "natural",
+                       "mix": "80% single-root BFS, 20% CC", "requests_per_rate": N,
+                       "workers": a.serve_workers, "max_batch": a.serve_batch, "results": "device buffers"},
+            "rates": rows, "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +209,14 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
+    ap.add_argument("--serve-scales", default="", help="serve: comma-separated R-MAT scales (default 18..25)")
+    ap.add_argument("--serve-grids", default="", help="serve: comma-separated grid sides (default 2048,4096)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rates", type=lambda s: [int(x) for x in s.split(",")], default=None)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--serve-buffers", type=int, default=48, help="serve: device result buffers per graph")
+    ap.add_argument("--serve-seed", type=int, default=5)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding is a "
@@ -67,6 +224,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     ensure_built()
+    if a.workload == "serve":
+        print(json.dumps(serve_bench(a)), flush=True)
+        return
     import blest_amd as B
     from blest_amd import workloads as W
 

@@ -25,7 +25,7 @@ from . import _lib
 from ._lib import BlestError, check
 
 __all__ = ["CSC", "BVSS", "BitMatrix", "DeviceGraph", "BVSSBFSKernel", "BVSSCCKernel", "BVSSMBFSKernel",
-           "BFSResult", "CCResult", "MBFSResult", "Reorder", "BlestError", "device_opts", "GraphBatch", "Server"]
+           "BFSResult", "CCResult", "MBFSResult", "Reorder", "BlestError", "device_opts", "GraphBatch", "Server", "pinned_empty", "autotune_bfs"]
 
 
 def _u32p(a: np.ndarray):
@@ -279,6 +279,24 @@ class DeviceGraph:
     def n(self) -> int:
         return self.info()["n"]
 
+    def set_thresholds(self, alpha: float, beta: float, direction: int = 0) -> None:
+        check(self._lib.blest_device_graph_set_thresholds(self._h, alpha, beta, direction))
+
+
+def autotune_bfs(bvss: BVSS, csc: CSC | None, inv_perm, roots, **base) -> tuple[dict, float, int]:
+    """Native autotuner: returns (best device opts, their mean BFS ms, candidates timed)."""
+    lib = _lib.load()
+    o = device_opts(**base)
+    best = _lib.DeviceOpts()
+    ms = C.c_double()
+    trials = C.c_uint32()
+    inv = None if inv_perm is None else _as_u32(inv_perm)
+    r = _as_u32(roots)
+    check(lib.blest_autotune_bfs(bvss._h, csc._h if csc is not None else None,
+                                 _u32p(inv) if inv is not None else None, _u32p(r), len(r), C.byref(o),
+                                 C.byref(best), C.byref(ms), C.byref(trials)))
+    return {k: getattr(best, k) for k, _ in best._fields_}, ms.value, trials.value
+
 
 class _Query:
     def __init__(self, dg: DeviceGraph, max_sources: int = 1):
@@ -485,9 +503,14 @@ class Server:
             self._lib.blest_server_free(h)
             self._h = C.c_void_p(0)
 
-    def submit(self, graph: int, kind: str = "bfs", root: int = 0) -> int:
+    def submit(self, graph: int, kind: str = "bfs", root: int = 0, out: np.ndarray | None = None) -> int:
+        """out: optional caller array of n int32 (BFS) / uint32 (CC), e.g. pinned_empty()."""
         k = self.KINDS[kind]
-        out = np.empty(self._n[graph], dtype=np.int32 if k == 0 else np.uint32)
+        want = np.int32 if k == 0 else np.uint32
+        if out is None:
+            out = np.empty(self._n[graph], dtype=want)
+        elif out.dtype != want or out.size < self._n[graph] or not out.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"out must be a contiguous {np.dtype(want).name} array of at least n entries")
         t = C.c_uint64()
         check(self._lib.blest_server_submit(self._h, graph, k, root, out.ctypes.data_as(C.c_void_p), C.byref(t)))
         self._out[t.value] = out
@@ -503,3 +526,26 @@ class Server:
 
     def drain(self) -> None:
         check(self._lib.blest_server_drain(self._h))
+
+
+class _Pinned:
+    def __init__(self, nbytes: int):
+        self._lib = _lib.load()
+        p = C.c_void_p()
+        check(self._lib.blest_host_alloc(nbytes, C.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self._lib.blest_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(n: int, dtype=np.int32) -> np.ndarray:
+    """A numpy array in page-locked host memory (freed with the array)."""
+    dt = np.dtype(dtype)
+    holder = _Pinned(max(n, 1) * dt.itemsize)
+    buf = (C.c_char * (max(n, 1) * dt.itemsize)).from_address(holder.ptr.value)
+    arr = np.frombuffer(buf, dtype=dt, count=n)
+    buf._holder = holder  # the ctypes buffer (referenced by the array) keeps the allocation alive
+    return arr

@@ -129,6 +129,11 @@ PROTOTYPES = {
     "blest_query_levels_device": (P, [P]),
     "blest_mbfs": (C.c_int, [P, U32P, C.c_uint32, I32P, C.POINTER(MbfsOpts), C.POINTER(BfsStats)]),
     "blest_cc": (C.c_int, [P, C.c_int, U32P, C.POINTER(CcStats)]),
+    "blest_device_graph_set_thresholds": (C.c_int, [P, C.c_double, C.c_double, C.c_int]),
+    "blest_autotune_bfs": (C.c_int, [P, P, U32P, U32P, C.c_uint32, C.POINTER(DeviceOpts), C.POINTER(DeviceOpts),
+                                     DP, U32P]),
+    "blest_host_alloc": (C.c_int, [C.c_size_t, PP]),
+    "blest_host_free": (None, [P]),
     "blest_device_memory_info": (C.c_int, [C.c_int, U64P, U64P]),
     "blest_query_footprint": (C.c_int, [P, C.c_uint32, U64P]),
     "blest_server_opts_default": (None, [C.POINTER(ServerOpts)]),

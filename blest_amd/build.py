@@ -21,7 +21,7 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = os.environ.get("BLEST_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["csc.cpp", "reorder.cpp", "bvss.cpp", "capi_host.cpp", "server.cpp"]
+HOST_SOURCES = ["csc.cpp", "reorder.cpp", "bvss.cpp", "capi_host.cpp", "server.cpp", "autotune.cpp"]
 DEVICE_SOURCES = ["device/dgraph.hip", "device/bfs.hip", "device/cc.hip", "device/mbfs.hip", "device/batch.hip"]
 CLI_SOURCE = "cli/blest_main.cpp"
 

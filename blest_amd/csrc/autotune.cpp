@@ -1,0 +1,104 @@
+// Launch-shape and direction-threshold autotuner for single-source BFS, built on the
+// C ABI.  Coordinate descent: segments per CU (a device-graph layout choice, so each
+// candidate is a fresh upload), then alpha, then beta (set in place).  Each candidate
+// is timed as the mean HIP-event time over the given roots after one warm-up pass;
+// results do not depend on the choice (levels are unique), only the time does.
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "blest.h"
+#include "capi_internal.hpp"
+
+namespace blest {
+namespace {
+
+struct Handles {
+    blest_device_graph* dg = nullptr;
+    blest_query* q = nullptr;
+    ~Handles() {
+        if (q) blest_query_free(q);
+        if (dg) blest_device_graph_free(dg);
+    }
+};
+
+void check(blest_status s) {
+    if (s != BLEST_OK) throw Error(s, blest_last_error());
+}
+
+double time_roots(blest_query* q, const uint32_t* roots, uint32_t nroots) {
+    blest_bfs_stats st{};
+    for (uint32_t i = 0; i < nroots; ++i) check(blest_bfs(q, roots[i], nullptr, &st));  // warm-up
+    double sum = 0;
+    for (uint32_t i = 0; i < nroots; ++i) {
+        check(blest_bfs(q, roots[i], nullptr, &st));
+        sum += st.kernel_ms;
+    }
+    return sum / nroots;
+}
+
+}  // namespace
+}  // namespace blest
+
+using namespace blest;
+
+extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc* g, const uint32_t* perm,
+                                           const uint32_t* roots, uint32_t nroots, const blest_device_opts* base,
+                                           blest_device_opts* best, double* best_ms, uint32_t* trials) {
+    return guarded([&] {
+        if (!b || !roots || !best) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        if (nroots == 0) throw Error(BLEST_ERR_INVALID_ARGUMENT, "the autotuner needs at least one root");
+        blest_device_opts o;
+        blest_device_opts_default(&o);
+        if (base) o = *base;
+        if (!g) o.keep_csr = 0;
+        uint32_t runs = 0;
+        double bestMs = std::numeric_limits<double>::infinity();
+        blest_device_opts bestOpts = o;
+        Handles keep;  // graph of the best layout so far
+        for (uint32_t bpc : {1u, 2u, 4u, 8u}) {
+            blest_device_opts c = o;
+            c.blocks_per_cu = bpc;
+            auto h = std::make_unique<Handles>();
+            check(blest_device_graph_create(b, g, perm, &c, &h->dg));
+            check(blest_query_create(h->dg, 1, &h->q));
+            const double ms = time_roots(h->q, roots, nroots);
+            ++runs;
+            if (ms < bestMs) {
+                bestMs = ms;
+                bestOpts = c;
+                std::swap(keep.dg, h->dg);
+                std::swap(keep.q, h->q);
+            }
+        }
+        if (bestOpts.direction == 0 && keep.dg) {
+            static const double alphas[] = {4, 8, 15, 30, 60, 120};
+            static const double betas[] = {6, 12, 18, 24, 48, 96};
+            for (double a : alphas) {
+                if (a == bestOpts.alpha) continue;
+                check(blest_device_graph_set_thresholds(keep.dg, a, bestOpts.beta, 0));
+                const double ms = time_roots(keep.q, roots, nroots);
+                ++runs;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestOpts.alpha = a;
+                }
+            }
+            for (double be : betas) {
+                if (be == bestOpts.beta) continue;
+                check(blest_device_graph_set_thresholds(keep.dg, bestOpts.alpha, be, 0));
+                const double ms = time_roots(keep.q, roots, nroots);
+                ++runs;
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestOpts.beta = be;
+                }
+            }
+        }
+        *best = bestOpts;
+        if (best_ms) *best_ms = bestMs;
+        if (trials) *trials = runs;
+    });
+}

@@ -349,7 +349,8 @@ void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st) {
     BLEST_HIP(hipStreamSynchronize(q.stream));
     q.pending = false;
     dev::Ctrl c;
-    BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
+    BLEST_HIP(hipMemcpyAsync(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipStreamSynchronize(q.stream));
     q.levelHint = c.levels;
     float ms = 0;
     BLEST_HIP(hipEventElapsedTime(&ms, q.ev0, q.ev1));
@@ -370,7 +371,7 @@ void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st) {
             BLEST_HIP(hipGetLastError());
             src = q.levelsOut;
         }
-        BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost, q.stream));
+        BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * g.n, hipMemcpyDefault, q.stream));
         BLEST_HIP(hipStreamSynchronize(q.stream));
     }
 }

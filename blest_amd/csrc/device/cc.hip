@@ -294,7 +294,8 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
     BLEST_HIP(hipStreamSynchronize(q.stream));
     if (algo == 1) {
         dev::Ctrl c;
-        BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
+        BLEST_HIP(hipMemcpyAsync(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipStreamSynchronize(q.stream));
         iterations = c.iterations;
     }
     float ms = 0;
@@ -315,7 +316,7 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
         tmp.resize(g.n);
         dst = tmp.data();
     }
-    BLEST_HIP(hipMemcpyAsync(dst, result, 4 * size_t(g.n), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipMemcpyAsync(dst, result, 4 * size_t(g.n), hipMemcpyDefault, q.stream));
     BLEST_HIP(hipStreamSynchronize(q.stream));
     if (st) {
         uint32_t comps = 0;

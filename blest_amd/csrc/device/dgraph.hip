@@ -191,6 +191,19 @@ blest_status blest_device_graph_info_get(const blest_device_graph* h, blest_devi
     });
 }
 
+blest_status blest_device_graph_set_thresholds(blest_device_graph* h, double alpha, double beta, int direction) {
+    return guarded([&] {
+        if (!h) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        if (!(alpha > 0) || !(beta > 0)) throw Error(BLEST_ERR_INVALID_ARGUMENT, "alpha and beta must be positive");
+        if (direction < 0 || direction > 2) throw Error(BLEST_ERR_INVALID_ARGUMENT, "direction must be 0, 1 or 2");
+        if (direction != 1 && !h->g.hasCsr)
+            throw Error(BLEST_ERR_UNSUPPORTED, "bottom-up needs the CSR (keep_csr at creation)");
+        h->g.opts.alpha = alpha;
+        h->g.opts.beta = beta;
+        h->g.opts.direction = direction;
+    });
+}
+
 blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, blest_query** out) {
     return guarded([&] {
         if (!dg || !out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
@@ -229,6 +242,22 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
 }
 
 void blest_query_free(blest_query* q) { delete q; }
+
+blest_status blest_host_alloc(size_t bytes, void** out) {
+    return guarded([&] {
+        if (!out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        if (hipHostMalloc(out, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            throw Error(BLEST_ERR_OUT_OF_HOST_MEMORY, "pinned host allocation failed");
+        }
+    });
+}
+
+void blest_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
 
 blest_status blest_device_memory_info(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
     return guarded([&] {

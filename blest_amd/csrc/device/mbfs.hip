@@ -426,7 +426,8 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     BLEST_HIP(hipStreamSynchronize(q.stream));
     dev::Ctrl c;
-    BLEST_HIP(hipMemcpy(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost));
+    BLEST_HIP(hipMemcpyAsync(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipStreamSynchronize(q.stream));
     q.levelHint = c.levels;
     float ms = 0;
     BLEST_HIP(hipEventElapsedTime(&ms, q.ev0, q.ev1));
@@ -448,12 +449,12 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
             src = q.levelsOut;
         }
         if (hostLevels)
-            BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * size_t(g.n) * k, hipMemcpyDeviceToHost,
+            BLEST_HIP(hipMemcpyAsync(hostLevels, src, sizeof(int32_t) * size_t(g.n) * k, hipMemcpyDefault,
                                      q.stream));
         else
             for (uint32_t j = 0; j < k; ++j)
                 BLEST_HIP(hipMemcpyAsync(rows[j], src + size_t(g.n) * j, sizeof(int32_t) * size_t(g.n),
-                                         hipMemcpyDeviceToHost, q.stream));
+                                         hipMemcpyDefault, q.stream));
         BLEST_HIP(hipStreamSynchronize(q.stream));
     }
 }

@@ -173,8 +173,8 @@ typedef struct blest_query blest_query;
 
 typedef struct blest_device_opts {
     int device;              /* HIP device ordinal */
-    uint32_t block_size;     /* threads per workgroup of persistent kernels (64..1024) */
-    uint32_t blocks_per_cu;  /* 0 = as many as occupancy allows */
+    uint32_t block_size;     /* informational: the level kernels run 512-thread workgroups */
+    uint32_t blocks_per_cu;  /* 1..8: frontier segments (= workgroups per level kernel) per CU */
     double alpha;            /* top-down -> bottom-up when frontier edges > unexplored edges / alpha */
     double beta;             /* bottom-up -> top-down when frontier vertices < n / beta */
     int direction;           /* 0 = optimizing, 1 = top-down only, 2 = bottom-up after level 1 */
@@ -199,6 +199,20 @@ typedef struct blest_device_graph_info {
 } blest_device_graph_info;
 blest_status blest_device_graph_info_get(const blest_device_graph* dg, blest_device_graph_info* i);
 
+/* Change the direction-switch thresholds (alpha, beta, direction) of a device graph;
+ * the layout fields of its opts are fixed at creation.  Not concurrently with a
+ * traversal on the graph. */
+blest_status blest_device_graph_set_thresholds(blest_device_graph* dg, double alpha, double beta, int direction);
+
+/* Autotuner (no reference counterpart; the reference hard-codes its launch shape):
+ * coordinate descent over blocks_per_cu (1, 2, 4, 8), then alpha, then beta, each
+ * candidate timed as the mean HIP-event time of single-source BFS from `roots`
+ * (original IDs) after one warm-up pass.  base (nullable) supplies the other fields;
+ * best receives the fastest opts, best_ms its mean time, trials the candidates run. */
+blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc* g, const uint32_t* perm_old_to_new,
+                                const uint32_t* roots, uint32_t nroots, const blest_device_opts* base,
+                                blest_device_opts* best, double* best_ms, uint32_t* trials);
+
 /* Per-query device state (levels, bitmaps, frontier lists).  max_sources bounds
  * blest_mbfs batches issued through this query. */
 blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, blest_query** out);
@@ -214,7 +228,9 @@ typedef struct blest_bfs_stats {
 } blest_bfs_stats;
 
 /* BFSKernel::run(src) (BFS/BFSKernel.cuh:53-56): single-source BFS.  levels[n]
- * (host memory, nullable) receives hop distances, -1 = unreachable. */
+ * (nullable) receives hop distances, -1 = unreachable.  Every result array of this
+ * header may be host memory (pageable or from blest_host_alloc) or device memory of
+ * the graph's device; the copy is ordered on the query's stream and complete on return. */
 blest_status blest_bfs(blest_query* q, uint32_t src, int32_t* levels, blest_bfs_stats* st);
 /* Same traversal without the device-to-host copy: the levels stay in the query's
  * device buffer (blest_query_levels_device). stream: hipStream_t or NULL. */
@@ -245,6 +261,10 @@ typedef struct blest_cc_stats {
 } blest_cc_stats;
 blest_status blest_cc(blest_query* q, int algo, uint32_t* labels, blest_cc_stats* st);
 
+/* Page-locked host memory (hipHostMalloc): result arrays placed here come back from the
+ * device at full link speed instead of through a staging copy. */
+blest_status blest_host_alloc(size_t bytes, void** out);
+void blest_host_free(void* p);
 /* Free and total HBM of a device (hipMemGetInfo). */
 blest_status blest_device_memory_info(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Device bytes one query needs for batches of up to max_sources roots. */
