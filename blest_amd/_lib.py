@@ -41,7 +41,7 @@ class BvssInfo(C.Structure):
 class DeviceOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("alpha", C.c_double), ("beta", C.c_double), ("direction", C.c_int), ("keep_csr", C.c_int),
-                ("timeout_ms", C.c_uint32)]
+                ("timeout_ms", C.c_uint32), ("use_graphs", C.c_int)]
 
 
 class DeviceGraphInfo(C.Structure):

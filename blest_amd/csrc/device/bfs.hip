@@ -47,6 +47,7 @@ struct BfsArgs {
     uint32_t src;
     int direction;
     double alpha, beta;
+    int levelFromCtrl;  // 1: the kernel's L argument is relative to ctrl->levelBase
 };
 
 __device__ __forceinline__ uint32_t front_byte(const unsigned long long* f, uint32_t r) {
@@ -93,7 +94,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
     }
 }
 
-__global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g, BfsArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+    const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
     __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
@@ -197,7 +199,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
     }
 }
 
-__global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+    const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
@@ -327,6 +330,9 @@ void bfs_launch(Query& q, uint32_t src) {
     a.direction = g.hasCsr ? g.opts.direction : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
+    a.levelFromCtrl = 0;
+    dev::BfsArgs ar = a;
+    ar.levelFromCtrl = 1;
     const GraphView gv = g.view();
     const dim3 grid = level_grid(g), block(dev::kProducerBlock);
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
@@ -337,7 +343,13 @@ void bfs_launch(Query& q, uint32_t src) {
             hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, a, L);
         },
-        [&] { hipLaunchKernelGGL(dev::bfs_final_kernel, grid, block, 0, q.stream, gv, a); });
+        [&] { hipLaunchKernelGGL(dev::bfs_final_kernel, grid, block, 0, q.stream, gv, a); },
+        &q.graphs[kGraphBfs],
+        [&](unsigned k) {
+            hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, ar, k);
+        },
+        launch_key(a.alpha, a.beta, a.direction, 0));
     BLEST_HIP(hipGetLastError());
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     q.lastSource = src;

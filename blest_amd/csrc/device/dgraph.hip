@@ -1,5 +1,7 @@
 // HBM-resident device graph and per-query state: the C ABI handles
 // blest_device_graph / blest_query, and the host-side level-loop driver.
+#include <cstring>
+
 #include "dgraph.hpp"
 #include "segments.hpp"
 
@@ -12,6 +14,8 @@ DeviceGraph::~DeviceGraph() {
 
 Query::~Query() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& lg : graphs)
+        if (lg.exec) (void)hipGraphExecDestroy(lg.exec);
     for (void* p : allocations) (void)hipFree(p);
     if (hostDone) (void)hipHostFree(hostDone);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -26,20 +30,82 @@ dev::SegView DeviceGraph::seg_view() const { return dev::SegView{segs, wps, segB
 
 dev::SegCounters Query::seg_counters() const { return dev::SegCounters{segCount, segNew, segEdges}; }
 
-void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail) {
+namespace dev {
+// Closes a replayed chunk of `count` levels: the next chunk starts after it.
+__global__ void advance_levels_kernel(Ctrl* c, unsigned count) {
+    if (threadIdx.x == 0) c->levelBase += count;
+}
+}  // namespace dev
+
+uint64_t launch_key(double alpha, double beta, int direction, uint32_t extra) {
+    uint64_t a, b;
+    std::memcpy(&a, &alpha, 8);
+    std::memcpy(&b, &beta, 8);
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (uint64_t x : {a, b, uint64_t(uint32_t(direction)), uint64_t(extra)}) h = (h ^ x) * 0x100000001B3ull + 0x632BE59BD9B4E019ull;
+    return h | 1;  // never 0 (0 = not captured)
+}
+
+static void capture_levels(Query& q, LevelGraph& lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
+    if (lg.exec) {
+        BLEST_HIP(hipGraphExecDestroy(lg.exec));
+        lg.exec = nullptr;
+    }
+    const unsigned C = kGraphChunkLevels;
+    BLEST_HIP(hipStreamBeginCapture(q.stream, hipStreamCaptureModeThreadLocal));
+    for (unsigned k = 0; k < C; ++k) levelRel(k);
+    hipLaunchKernelGGL(dev::advance_levels_kernel, dim3(1), dim3(64), 0, q.stream, q.ctrl, C);
+    hipGraph_t graph = nullptr;
+    const hipError_t launchErr = hipGetLastError();
+    const hipError_t endErr = hipStreamEndCapture(q.stream, &graph);
+    if (launchErr != hipSuccess || endErr != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        throw Error(BLEST_ERR_DEVICE, std::string("level graph capture failed: ") +
+                                          hipGetErrorString(launchErr != hipSuccess ? launchErr : endErr));
+    }
+    const hipError_t instErr = hipGraphInstantiate(&lg.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (instErr != hipSuccess) {
+        lg.exec = nullptr;
+        throw Error(BLEST_ERR_DEVICE, std::string("level graph instantiation failed: ") + hipGetErrorString(instErr));
+    }
+    lg.chunk = C;
+    lg.key = key;
+}
+
+static bool poll_done(Query& q, const std::function<void()>& tail) {
+    // The tail (e.g. marking unreached vertices) is idempotent and harmless before the
+    // end: enqueuing it ahead of the poll hides the host round trip when the chunk was
+    // the last one.
+    if (tail) tail();
+    BLEST_HIP(hipMemcpyAsync(q.hostDone, &q.ctrl->done, sizeof(unsigned), hipMemcpyDeviceToHost, q.stream));
+    BLEST_HIP(hipEventRecord(q.evChunk, q.stream));
+    BLEST_HIP(hipEventSynchronize(q.evChunk));
+    return *q.hostDone != 0;
+}
+
+void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail,
+                LevelGraph* lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
+    const unsigned limit = q.g->n + 2;
+    const int mode = q.g->opts.use_graphs;
+    if (lg && levelRel && (mode == 1 || (mode == 0 && q.levelHint >= kGraphMinLevels))) {
+        if (!lg->exec || lg->key != key) capture_levels(q, *lg, levelRel, key);
+        const unsigned C = lg->chunk;
+        unsigned launched = 0;
+        unsigned reps = std::max(1u, (q.levelHint + 2 + C - 1) / C);
+        for (;;) {
+            for (unsigned i = 0; i < reps && launched <= limit; ++i, launched += C)
+                BLEST_HIP(hipGraphLaunch(lg->exec, q.stream));
+            if (poll_done(q, tail)) return;
+            if (launched > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
+            reps = std::min(reps * 2, std::max(1u, 4096u / C));
+        }
+    }
     unsigned L = 1;
     unsigned chunk = std::max(4u, q.levelHint + 2);
-    const unsigned limit = q.g->n + 2;
     for (;;) {
         for (unsigned i = 0; i < chunk && L <= limit; ++i) level(L++);
-        // The tail (e.g. marking unreached vertices) is idempotent and harmless before
-        // the end: enqueuing it ahead of the poll hides the host round trip when the
-        // chunk was the last one.
-        if (tail) tail();
-        BLEST_HIP(hipMemcpyAsync(q.hostDone, &q.ctrl->done, sizeof(unsigned), hipMemcpyDeviceToHost, q.stream));
-        BLEST_HIP(hipEventRecord(q.evChunk, q.stream));
-        BLEST_HIP(hipEventSynchronize(q.evChunk));
-        if (*q.hostDone) break;
+        if (poll_done(q, tail)) break;
         if (L > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
         chunk = std::min(chunk * 2, 4096u);
     }
@@ -66,6 +132,7 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->direction = 0;
     o->keep_csr = 1;
     o->timeout_ms = 20000;
+    o->use_graphs = 0;
 }
 
 blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,
@@ -82,6 +149,7 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
         if (opts.blocks_per_cu == 0 || opts.blocks_per_cu > 8)
             throw Error(BLEST_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 8]");
         if (opts.keep_csr && !gh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "keep_csr needs the CSC");
+        if (opts.use_graphs < 0 || opts.use_graphs > 2) throw Error(BLEST_ERR_INVALID_ARGUMENT, "use_graphs must be 0, 1 or 2");
         auto* h = new blest_device_graph();
         DeviceGraph& g = h->g;
         try {

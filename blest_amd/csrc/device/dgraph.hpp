@@ -91,6 +91,18 @@ struct DeviceGraph {
     ~DeviceGraph();
 };
 
+// A captured chunk of levels (kernels that read their level from ctrl->levelBase),
+// replayed until the traversal finishes.  key = the launch parameters it was captured
+// with; a different key recaptures.
+struct LevelGraph {
+    hipGraphExec_t exec = nullptr;
+    unsigned chunk = 0;
+    uint64_t key = 0;
+};
+enum LevelGraphKind { kGraphBfs = 0, kGraphMbfs1 = 1, kGraphMbfs2 = 2, kGraphCount = 3 };
+constexpr unsigned kGraphChunkLevels = 32;  // levels per replayed graph
+constexpr unsigned kGraphMinLevels = 64;    // auto mode: replay once a traversal was this deep
+
 // Mutable per-query device state.
 struct Query {
     DeviceGraph* g = nullptr;
@@ -113,6 +125,7 @@ struct Query {
     uint32_t mbfsWords = 0;
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
+    LevelGraph graphs[kGraphCount];
     std::vector<void*> allocations;
     std::mutex mu;
     uint32_t lastSource = 0;
@@ -128,11 +141,15 @@ struct Query {
     ~Query();
 };
 
-// Level-loop driver (bfs.hip): enqueues level(L) for L = 1, 2, ... in chunks and polls
-// the device's done flag after each chunk.
+// Level-loop driver: enqueues level(L) for L = 1, 2, ... in chunks and polls the
+// device's done flag after each chunk.  With lg and levelRel given (levelRel(k) launches
+// level levelBase + k), deep traversals replay a captured chunk of levels instead
+// (device opts use_graphs: 0 auto, 1 always, 2 never).
 dim3 level_grid(const DeviceGraph& g);
-void run_levels(Query& q, const std::function<void(unsigned)>& level,
-                const std::function<void()>& tail = nullptr);
+void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail = nullptr,
+                LevelGraph* lg = nullptr, const std::function<void(unsigned)>& levelRel = nullptr,
+                uint64_t key = 0);
+uint64_t launch_key(double alpha, double beta, int direction, uint32_t extra);
 uint32_t map_source(const DeviceGraph& g, uint32_t src);
 void bfs_launch(Query& q, uint32_t srcReordered);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);

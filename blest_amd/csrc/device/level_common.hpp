@@ -71,8 +71,15 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
 
 __device__ __forceinline__ void level_state_reset(Ctrl* c) {
     c->done = 0;
+    c->levelBase = 1;
     c->state[0] = LevelState{0, 0, 0, 0, 0, 0, 0ull, 0ull};
     c->state[1] = c->state[0];
+}
+
+// Level of a kernel: fixed at launch, or (in a replayed chunk of levels) the chunk's
+// base level held in ctrl plus the kernel's position in the chunk.
+__device__ __forceinline__ unsigned level_of(const Ctrl* c, int fromCtrl, unsigned L) {
+    return fromCtrl ? __hip_atomic_load(&c->levelBase, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + L : L;
 }
 
 }  // namespace dev

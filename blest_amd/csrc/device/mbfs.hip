@@ -37,6 +37,7 @@ struct MbfsArgs {
     SegCounters sc;
     int direction;
     double alpha, beta;
+    int levelFromCtrl;  // 1: the kernel's L argument is relative to ctrl->levelBase
 };
 
 template <int NW>
@@ -152,7 +153,8 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphVi
 }
 
 template <int NW>
-__global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
+    const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
     __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
@@ -286,7 +288,8 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
 }
 
 template <int NW>
-__global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned L) {
+__global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
+    const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
@@ -382,13 +385,21 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     hipLaunchKernelGGL(dev::mbfs_init_kernel<NW>, grid, block, 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_seed_kernel<NW>, dim3(2), dim3(64), 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_first_list_kernel<NW>, grid, block, 0, q.stream, gv, a);
+    dev::MbfsArgs ar = a;
+    ar.levelFromCtrl = 1;
     run_levels(
         q,
         [&](unsigned L) {
             hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
         },
-        [&] { hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, grid, block, 0, q.stream, gv, a); });
+        [&] { hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, grid, block, 0, q.stream, gv, a); },
+        &q.graphs[NW == 1 ? kGraphMbfs1 : kGraphMbfs2],
+        [&](unsigned k) {
+            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
+        },
+        launch_key(a.alpha, a.beta, a.direction, a.k));
     BLEST_HIP(hipGetLastError());
 }
 
@@ -420,6 +431,7 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.direction = g.hasCsr ? g.opts.direction : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
+    a.levelFromCtrl = 0;
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
     if (nw == 1) mbfs_enqueue_levels<1>(q, a);
     else mbfs_enqueue_levels<2>(q, a);

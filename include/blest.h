@@ -180,6 +180,9 @@ typedef struct blest_device_opts {
     int direction;           /* 0 = optimizing, 1 = top-down only, 2 = bottom-up after level 1 */
     int keep_csr;            /* upload the bottom-up CSR (needed for direction != 1) */
     uint32_t timeout_ms;     /* bound on every in-kernel wait */
+    int use_graphs;          /* level chunks replayed from a captured HIP graph: 0 = once a
+                                query's previous traversal was >= 64 levels deep, 1 = always,
+                                2 = never */
 } blest_device_opts;
 void blest_device_opts_default(blest_device_opts* o);
 
