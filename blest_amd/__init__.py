@@ -506,6 +506,8 @@ class Server:
     def submit(self, graph: int, kind: str = "bfs", root: int = 0, out: np.ndarray | None = None) -> int:
         """out: optional caller array of n int32 (BFS) / uint32 (CC), e.g. pinned_empty()."""
         k = self.KINDS[kind]
+        if not 0 <= graph < len(self._n):
+            raise BlestError(1, "graph index out of range")
         want = np.int32 if k == 0 else np.uint32
         if out is None:
             out = np.empty(self._n[graph], dtype=want)

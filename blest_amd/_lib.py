@@ -48,7 +48,7 @@ class DeviceGraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("rsets", C.c_uint32), ("vsets", C.c_uint32), ("grid_blocks", C.c_uint32),
                 ("block_size", C.c_uint32), ("cus", C.c_uint32), ("symmetric", C.c_uint32),
                 ("has_csr", C.c_uint32), ("slices", C.c_uint64), ("nnz", C.c_uint64),
-                ("device_bytes", C.c_uint64), ("device", C.c_int32)]
+                ("device_bytes", C.c_uint64), ("device", C.c_int32), ("stream_quads", C.c_uint64)]
 
 
 class BfsStats(C.Structure):

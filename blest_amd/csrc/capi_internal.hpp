@@ -19,8 +19,10 @@ const BVSS& bvss_of(const blest_bvss* h);
 
 // Multi-source BFS writing the levels of root j into rows[j] (n entries each); used by
 // the request server to hand each request its own row without a staging copy.
+// perRoot (nullable, k entries) receives each root's own level count, visited count
+// and component edges.
 blest_status mbfs_rows(blest_query* q, const uint32_t* srcs, uint32_t k, int32_t* const* rows,
-                       blest_bfs_stats* st);
+                       blest_bfs_stats* st, blest_bfs_stats* perRoot);
 
 // Runs fn, converting exceptions into a status code (and the thread's last error).
 template <class F>

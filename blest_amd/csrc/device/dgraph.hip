@@ -166,8 +166,7 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
             g.V = b.noVirtualSliceSets;
             g.W = (b.n + 63) / 64;
             g.S = b.noSlices;
-            g.Q = b.noSlices / 4;
-            if (g.Q > 0xFFFFFFFFull) throw Error(BLEST_ERR_UNSUPPORTED, "more than 2^32 slice quads");
+            if (b.noSlices / 4 > 0xFFFFFFFFull) throw Error(BLEST_ERR_UNSUPPORTED, "more than 2^32 slice quads");
             // Segments: one producer workgroup per contiguous run of bitmap words.
             g.segs = std::min<uint32_t>(std::min<uint32_t>(uint32_t(g.cus) * g.blocksPerCu, dev::kMaxSegs), g.W);
             g.wps = (g.W + g.segs - 1) / g.segs;
@@ -182,11 +181,38 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
                 g.segBase = g.alloc<uint32_t>(sb.size());
                 BLEST_HIP(hipMemcpy(g.segBase, sb.data(), 4 * sb.size(), hipMemcpyHostToDevice));
             }
-            std::vector<uint32_t> uq(size_t(g.V) + 1);
-            for (size_t v = 0; v <= g.V; ++v) {
+            // Device slice stream: the real slices of each vset packed four to a quad.  The
+            // reference layout's padding slots (row 0xFFFFFFFF with an empty mask; a
+            // full-padding vset has 128 slots whatever its slice count) carry no edge, and
+            // a traversal does not depend on the order of the slices inside a vset, so
+            // the device streams real slices only.  Byte s of the mask words is the mask
+            // of slot s (slot = lane * 4 + position).
+            const uint8_t* slotMask = reinterpret_cast<const uint8_t*>(b.masks.data());
+            for (size_t v = 0; v <= g.V; ++v)
                 if (b.sliceSetPtrs[v] % 4) throw Error(BLEST_ERR_INVALID_ARGUMENT, "vset not a multiple of 4 slices");
-                uq[v] = uint32_t(b.sliceSetPtrs[v] / 4);
-            }
+            std::vector<uint32_t> uq(size_t(g.V) + 1, 0);
+            const unsigned threads = hw_threads();
+            parallel_blocks(g.V, threads, [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t v = lo; v < hi; ++v) {
+                    uint32_t real = 0;
+                    for (uint64_t sl = b.sliceSetPtrs[v]; sl < b.sliceSetPtrs[v + 1]; ++sl) real += slotMask[sl] != 0;
+                    uq[v + 1] = (real + 3) / 4;
+                }
+            });
+            for (size_t v = 0; v < g.V; ++v) uq[v + 1] += uq[v];
+            g.Q = uq[g.V];
+            std::vector<uint32_t> pRows(size_t(g.Q) * 4, 0xFFFFFFFFu), pMasks(g.Q, 0);
+            parallel_blocks(g.V, threads, [&](uint64_t lo, uint64_t hi, unsigned) {
+                for (uint64_t v = lo; v < hi; ++v) {
+                    uint64_t o = uint64_t(uq[v]) * 4;
+                    for (uint64_t sl = b.sliceSetPtrs[v]; sl < b.sliceSetPtrs[v + 1]; ++sl) {
+                        if (!slotMask[sl]) continue;
+                        pRows[o] = b.rowIds[sl];
+                        pMasks[o / 4] |= uint32_t(slotMask[sl]) << (8 * (o % 4));
+                        ++o;
+                    }
+                }
+            });
             g.unitQuad = g.alloc<uint32_t>(uq.size());
             BLEST_HIP(hipMemcpy(g.unitQuad, uq.data(), 4 * uq.size(), hipMemcpyHostToDevice));
             g.unitRset = g.alloc<uint32_t>(g.V);
@@ -194,9 +220,9 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
             g.rsetUnit = g.alloc<uint32_t>(size_t(g.R) + 1);
             BLEST_HIP(hipMemcpy(g.rsetUnit, b.realPtrs.data(), 4 * (size_t(g.R) + 1), hipMemcpyHostToDevice));
             g.rows = g.alloc<uint4>(g.Q);
-            BLEST_HIP(hipMemcpy(g.rows, b.rowIds.data(), 16 * g.Q, hipMemcpyHostToDevice));
+            if (g.Q) BLEST_HIP(hipMemcpy(g.rows, pRows.data(), 16 * size_t(g.Q), hipMemcpyHostToDevice));
             g.masks = g.alloc<uint32_t>(g.Q);
-            BLEST_HIP(hipMemcpy(g.masks, b.masks.data(), 4 * g.Q, hipMemcpyHostToDevice));
+            if (g.Q) BLEST_HIP(hipMemcpy(g.masks, pMasks.data(), 4 * size_t(g.Q), hipMemcpyHostToDevice));
             if (gh) {
                 const CSC& c = csc_of(gh);
                 if (c.n != b.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "CSC and BVSS disagree on n");
@@ -256,6 +282,7 @@ blest_status blest_device_graph_info_get(const blest_device_graph* h, blest_devi
         i->nnz = g.nnz;
         i->device_bytes = g.deviceBytes;
         i->device = g.device;
+        i->stream_quads = g.Q;
     });
 }
 

@@ -125,6 +125,7 @@ struct Query {
     uint32_t mbfsWords = 0;
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
+    void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries
     LevelGraph graphs[kGraphCount];
     std::vector<void*> allocations;
     std::mutex mu;

@@ -363,6 +363,57 @@ __global__ void unpermute_rows_kernel(int32_t* out, const int32_t* in, const uin
             out[size_t(r) * n + i] = in[size_t(r) * n + oldToNew[i]];
 }
 
+// Per-root totals of a multi-source run (rows of k x n levels, -1 = unreached).
+struct RootStats {
+    unsigned levels;  // max level + 1
+    unsigned visited;
+    unsigned long long degreeSum;
+};
+
+__global__ void root_stats_clear_kernel(RootStats* st, uint32_t k) {
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) st[i] = RootStats{0u, 0u, 0ull};
+}
+
+__global__ void __launch_bounds__(256) root_stats_kernel(RootStats* st, const int32_t* levels, const uint32_t* csrPtr,
+                                                         uint32_t n) {
+    __shared__ unsigned sMax[4], sCnt[4];
+    __shared__ unsigned long long sDeg[4];
+    const uint32_t r = blockIdx.y;
+    const int32_t* row = levels + size_t(r) * n;
+    unsigned mx = 0, cnt = 0;
+    unsigned long long deg = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t l = row[i];
+        if (l >= 0) {
+            mx = max(mx, unsigned(l) + 1);
+            ++cnt;
+            if (csrPtr) deg += csrPtr[i + 1] - csrPtr[i];
+        }
+    }
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        mx = max(mx, unsigned(__shfl_xor(int(mx), d, kWave)));
+        cnt += __shfl_xor(cnt, d, kWave);
+        deg += __shfl_xor(deg, d, kWave);
+    }
+    const unsigned w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        sMax[w] = mx;
+        sCnt[w] = cnt;
+        sDeg[w] = deg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned i = 1; i < blockDim.x / kWave; ++i) {
+            mx = max(mx, sMax[i]);
+            cnt += sCnt[i];
+            deg += sDeg[i];
+        }
+        if (mx) atomicMax(&st[r].levels, mx);
+        if (cnt) atomicAdd(&st[r].visited, cnt);
+        if (deg) atomicAdd(&st[r].degreeSum, deg);
+    }
+}
+
 }  // namespace dev
 
 static void mbfs_ensure_state(Query& q, uint32_t nw) {
@@ -374,6 +425,7 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     q.mFront1 = q.alloc<unsigned long long>(words);
     q.mNext = q.alloc<unsigned long long>(words);
     if (!q.srcs) q.srcs = q.alloc<uint32_t>(128);
+    if (!q.rootStats) q.rootStats = q.alloc<dev::RootStats>(128);
     q.mbfsWords = nw;
 }
 
@@ -405,7 +457,7 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
 
 // hostLevels: k x n contiguous rows, or rows: k separate arrays of n (either may be null).
 static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLevels, int32_t* const* rows,
-                     const blest_mbfs_opts* o, blest_bfs_stats* st) {
+                     const blest_mbfs_opts* o, blest_bfs_stats* st, blest_bfs_stats* perRoot = nullptr) {
     DeviceGraph& g = *q.g;
     if (k == 0 || k > 128) throw Error(BLEST_ERR_INVALID_ARGUMENT, "multi-source batches hold 1..128 roots");
     if (k > q.maxSources) throw Error(BLEST_ERR_INVALID_ARGUMENT, "more roots than the query's max_sources");
@@ -452,6 +504,24 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
         st->edges_traversed_component = c.visitedEdges / 2;
         st->bottom_up_mask = c.buMask;
     }
+    if (perRoot) {
+        auto* rsDev = static_cast<dev::RootStats*>(q.rootStats);
+        hipLaunchKernelGGL(dev::root_stats_clear_kernel, dim3(1), dim3(128), 0, q.stream, rsDev, k);
+        const uint32_t bx = std::min<uint32_t>(256, (g.n + 255) / 256);
+        hipLaunchKernelGGL(dev::root_stats_kernel, dim3(bx, k), dim3(256), 0, q.stream, rsDev, q.levels,
+                           g.hasCsr ? g.csrPtr : nullptr, g.n);
+        BLEST_HIP(hipGetLastError());
+        std::vector<dev::RootStats> rs(k);
+        BLEST_HIP(hipMemcpyAsync(rs.data(), rsDev, sizeof(dev::RootStats) * k, hipMemcpyDeviceToHost, q.stream));
+        BLEST_HIP(hipStreamSynchronize(q.stream));
+        for (uint32_t j = 0; j < k; ++j) {
+            perRoot[j] = st ? *st : blest_bfs_stats{};
+            perRoot[j].kernel_ms = ms;
+            perRoot[j].total_levels = rs[j].levels;
+            perRoot[j].no_visited = rs[j].visited;
+            perRoot[j].edges_traversed_component = rs[j].degreeSum / 2;
+        }
+    }
     if (hostLevels || rows) {
         const int32_t* src = q.levels;
         if (g.hasPerm) {
@@ -487,12 +557,12 @@ extern "C" blest_status blest_mbfs(blest_query* qh, const uint32_t* srcs, uint32
 }
 
 blest_status blest::mbfs_rows(blest_query* qh, const uint32_t* srcs, uint32_t k, int32_t* const* rows,
-                              blest_bfs_stats* st) {
+                              blest_bfs_stats* st, blest_bfs_stats* perRoot) {
     return guarded([&] {
         if (!qh || !srcs || !rows) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
         Query& q = qh->q;
         std::lock_guard<std::mutex> lk(q.mu);
         BLEST_HIP(hipSetDevice(q.g->device));
-        mbfs_run(q, srcs, k, nullptr, rows, nullptr, st);
+        mbfs_run(q, srcs, k, nullptr, rows, nullptr, st, perRoot);
     });
 }

@@ -174,6 +174,7 @@ struct Server {
         blest_status st = BLEST_OK;
         std::string why;
         blest_bfs_stats bs{};
+        std::vector<blest_bfs_stats> perRoot(k);
         blest_cc_stats cs{};
         PooledQuery* pq = nullptr;
         try {
@@ -189,6 +190,7 @@ struct Server {
                 st = blest_cc(pq->q, 0, static_cast<uint32_t*>(batch[0]->out), &cs);
             } else if (k == 1) {
                 st = blest_bfs(pq->q, batch[0]->root, static_cast<int32_t*>(batch[0]->out), &bs);
+                perRoot[0] = bs;
             } else {
                 std::vector<uint32_t> roots(k);
                 std::vector<int32_t*> rows(k);
@@ -196,14 +198,15 @@ struct Server {
                     roots[j] = batch[j]->root;
                     rows[j] = static_cast<int32_t*>(batch[j]->out);
                 }
-                st = mbfs_rows(pq->q, roots.data(), k, rows.data(), &bs);
+                st = mbfs_rows(pq->q, roots.data(), k, rows.data(), &bs, perRoot.data());
             }
             if (st != BLEST_OK) why = blest_last_error();
         }
         const Clock::time_point end = Clock::now();
         std::lock_guard<std::mutex> lk(mu);
         if (pq) release(pq);
-        for (auto& r : batch) {
+        for (uint32_t j = 0; j < k; ++j) {
+            Request* r = batch[j].get();
             blest_request_result& res = r->res;
             res.ticket = r->ticket;
             res.status = st;
@@ -216,10 +219,10 @@ struct Server {
                 res.no_visited = cs.no_components;
                 res.edges_traversed_component = 0;
             } else {
-                res.kernel_ms = bs.kernel_ms;
-                res.total_levels = bs.total_levels;
-                res.no_visited = bs.no_visited;
-                res.edges_traversed_component = bs.edges_traversed_component;
+                res.kernel_ms = perRoot[j].kernel_ms;
+                res.total_levels = perRoot[j].total_levels;
+                res.no_visited = perRoot[j].no_visited;
+                res.edges_traversed_component = perRoot[j].edges_traversed_component;
             }
             r->error = why;
             r->done = true;

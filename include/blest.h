@@ -199,6 +199,7 @@ typedef struct blest_device_graph_info {
     uint32_t n, rsets, vsets, grid_blocks, block_size, cus, symmetric, has_csr;
     uint64_t slices, nnz, device_bytes;
     int32_t device;
+    uint64_t stream_quads;   /* device slice stream: real slices packed 4 per 20-byte quad */
 } blest_device_graph_info;
 blest_status blest_device_graph_info_get(const blest_device_graph* dg, blest_device_graph_info* i);
 
