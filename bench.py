@@ -196,6 +196,129 @@ def serve_bench(a) -> dict:
             "rates": rows, "roofline": None, "cpu_baseline": None}
 
 
+def mbfs_bench(a, prep) -> dict:
+    """Batched multi-source BFS against plain single-source runs on the same roots:
+    aggregate GTEPS = sum of m_c / sum of kernel time; speedup = batched / plain."""
+    import ctypes as C
+
+    import blest_amd as B
+    lib = B._lib.load()
+    dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm)
+    roots = [int(r) for r in prep.roots]
+    if a.mbfs_roots > len(roots):
+        # more seeded roots of degree >= 1, picked on the resident (reordered) graph and
+        # mapped back to original IDs
+        new_to_old = np.argsort(prep.inv_perm)
+        roots = [int(new_to_old[r]) for r in prep.csc.pick_roots(a.mbfs_roots, seed=prep.workload.root_seed)]
+    width = a.mbfs_width
+    q1 = B._Query(dg, 1)
+    st = B._lib.BfsStats()
+    for r in roots[:4]:
+        B.check(lib.blest_bfs(q1._h, r, None, C.byref(st)))
+    plain_ms, mc = [], []
+    for r in roots:
+        B.check(lib.blest_bfs(q1._h, r, None, C.byref(st)))
+        plain_ms.append(st.kernel_ms)
+        mc.append(st.edges_traversed_component)
+    del q1
+    qm = B._Query(dg, width)
+    o = B._lib.MbfsOpts(0, width)
+    batches = [np.ascontiguousarray(np.array(roots[i:i + width], dtype=np.uint32)) for i in range(0, len(roots), width)]
+    B.check(lib.blest_mbfs(qm._h, B._u32p(batches[0]), len(batches[0]), None, C.byref(o), C.byref(st)))  # warm-up
+    batch_ms = []
+    for _ in range(a.steps):
+        for b in batches:
+            B.check(lib.blest_mbfs(qm._h, B._u32p(b), len(b), None, C.byref(o), C.byref(st)))
+            batch_ms.append(st.kernel_ms)
+    total_mc = float(sum(mc))
+    plain = total_mc / (sum(plain_ms) * 1e-3) / 1e9
+    batched = total_mc * a.steps / (sum(batch_ms) * 1e-3) / 1e9
+    log(f"[mbfs] plain {plain:.2f} GTEPS ({np.mean(plain_ms):.3f} ms/root), batched {batched:.2f} GTEPS "
+        f"({np.mean(batch_ms):.3f} ms per {width}-root batch): {batched / plain:.2f}x")
+    info = dg.info()
+    return {"metric": f"batched MBFS aggregate GTEPS ({width} roots per launch)", "value": round(batched, 3),
+            "unit": "GTEPS", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+            "ms_per_step": round(float(sum(batch_ms)) / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": prep.workload.name + "_mbfs", "vertices": info["n"], "directed_edges": info["nnz"],
+                       "roots": len(roots), "batch": width, "reorder": prep.workload.reorder.name.lower()},
+            "plain_gteps": round(plain, 3), "speedup_vs_plain": round(batched / plain, 2),
+            "plain_ms_per_root": round(float(np.mean(plain_ms)), 4),
+            "batch_ms": round(float(np.mean(batch_ms)), 4), "roofline": None, "cpu_baseline": None}
+
+
+def batch_bench(a) -> dict:
+    """Many small graphs in one launch: 4,096 Erdos-Renyi graphs, n ~ U[1024, 65536]
+    (seeded), average degree 8, one seeded root each; aggregate GTEPS = sum of m_c over
+    graphs / launch time.  Plain = the same graphs one at a time through blest_bfs
+    (a seeded sample of graphs, scaled to the batch)."""
+    import ctypes as C
+
+    import blest_amd as B
+    lib = B._lib.load()
+    rng = np.random.default_rng(a.batch_seed)
+    count = a.batch_graphs
+    sizes = rng.integers(1024, 65537, size=count)
+    t0 = time.time()
+    graphs = [B.CSC.erdos_renyi(int(n), 8.0, seed=10_000 + i) for i, n in enumerate(sizes)]
+    roots = np.array([int(g.pick_roots(1, seed=i)[0]) for i, g in enumerate(graphs)], dtype=np.uint32)
+    log(f"[batch] {count} graphs, {int(sizes.sum())} vertices generated in {time.time() - t0:.0f}s")
+    t0 = time.time()
+    gb = B.GraphBatch(graphs)
+    log(f"[batch] uploaded in {time.time() - t0:.0f}s")
+    levels, _ = gb.bfs(roots)  # warm-up; also gives m_c
+    mc = 0
+    for g, lv in zip(graphs, levels):
+        cp, _ = g.arrays()
+        deg = np.diff(cp.astype(np.int64))
+        mc += int(deg[lv >= 0].sum()) // 2
+    ms = C.c_double()
+    times = []
+    for _ in range(a.steps):
+        B.check(lib.blest_graph_batch_bfs(gb._h, B._u32p(roots), None, C.byref(ms)))
+        times.append(ms.value)
+    t_batch = float(np.mean(times))
+    batched = mc / (t_batch * 1e-3) / 1e9
+    # plain: a seeded sample of the graphs one at a time
+    sample = rng.choice(count, size=min(count, a.batch_plain_sample), replace=False)
+    plain_ms, plain_mc = 0.0, 0
+    st = B._lib.BfsStats()
+    for i in sample:
+        g = graphs[int(i)]
+        bv = B.BVSS(8, 4).construct_from_csc(g)
+        dg = B.DeviceGraph(bv, g, None)
+        qq = B._Query(dg, 1)
+        B.check(lib.blest_bfs(qq._h, int(roots[i]), None, C.byref(st)))
+        B.check(lib.blest_bfs(qq._h, int(roots[i]), None, C.byref(st)))
+        plain_ms += st.kernel_ms
+        plain_mc += st.edges_traversed_component
+        del qq, dg
+    plain = plain_mc / (plain_ms * 1e-3) / 1e9
+    log(f"[batch] one launch {t_batch:.3f} ms: {batched:.2f} GTEPS; plain one at a time {plain:.3f} GTEPS "
+        f"({len(sample)} graphs sampled)")
+    return {"metric": f"graph-batch aggregate GTEPS ({count} graphs, one launch)", "value": round(batched, 3),
+            "unit": "GTEPS", "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": round(t_batch, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+            "config": {"workload": "er_batch", "graphs": count, "n_range": [1024, 65536], "avg_degree": 8,
+                       "vertices": int(sizes.sum()), "seed": a.batch_seed},
+            "plain_gteps": round(plain, 3), "speedup_vs_plain": round(batched / plain, 2),
+            "plain_sample": len(sample), "roofline": None, "cpu_baseline": None}
+
+
+def copy_calibration(a) -> dict:
+    """A plain device copy of known size (torch's copy kernel), run under the same HBM
+    byte counters as the traversal to scale them: counter bytes / (2 x copy bytes)."""
+    import torch
+    nbytes = a.copy_mib << 20
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(a.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"metric": "device copy", "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+            "copies": a.steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +340,13 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--serve-buffers", type=int, default=48, help="serve: device result buffers per graph")
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--mbfs", action="store_true", help="batched multi-source BFS vs plain on the workload's roots")
+    ap.add_argument("--mbfs-width", type=int, default=128)
+    ap.add_argument("--mbfs-roots", type=int, default=1024)
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-plain-sample", type=int, default=128)
+    ap.add_argument("--copy-mib", type=int, default=1024, help="copy: bytes of the calibration copy")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU sharding is a "
@@ -227,10 +357,19 @@ def main():
     if a.workload == "serve":
         print(json.dumps(serve_bench(a)), flush=True)
         return
+    if a.workload == "copy":
+        print(json.dumps(copy_calibration(a)), flush=True)
+        return
+    if a.workload == "er_batch":
+        print(json.dumps(batch_bench(a)), flush=True)
+        return
     import blest_amd as B
     from blest_amd import workloads as W
 
     prep = W.prepare(a.workload, log=log)
+    if a.mbfs:
+        print(json.dumps(mbfs_bench(a, prep)), flush=True)
+        return
     w = prep.workload
     opts = {}
     if a.block_size:

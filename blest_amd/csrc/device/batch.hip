@@ -25,6 +25,7 @@ struct BatchArgs {
     const uint32_t* csrPtr;
     const uint32_t* csrIdx;
     const uint32_t* roots;
+    const uint32_t* order;
     int32_t* levels;
     uint32_t maxWords;
     double alpha;
@@ -34,7 +35,8 @@ __global__ void __launch_bounds__(512) batch_bfs_kernel(BatchArgs a) {
     extern __shared__ unsigned long long lds[];
     __shared__ unsigned s_new;
     __shared__ unsigned long long s_edges;
-    const BatchGraph G = a.graphs[blockIdx.x];
+    const uint32_t gi = a.order[blockIdx.x];
+    const BatchGraph G = a.graphs[gi];
     const uint32_t W = (G.n + 63) / 64;
     unsigned long long* vis = lds;
     unsigned long long* front = lds + a.maxWords;
@@ -46,7 +48,7 @@ __global__ void __launch_bounds__(512) batch_bfs_kernel(BatchArgs a) {
     const uint32_t* ci = a.csrIdx + G.csrIdxOff;
     int32_t* levels = a.levels + G.levelOff;
     const unsigned lane = lane_id(), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
-    const uint32_t src = a.roots[blockIdx.x];
+    const uint32_t src = a.roots[gi];
     for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) {
         const unsigned long long x = (w == src / 64) ? (1ull << (src & 63)) : 0ull;
         vis[w] = x;
@@ -143,6 +145,7 @@ struct GraphBatch {
     std::vector<uint32_t> sizes;
     dev::BatchGraph* graphs = nullptr;
     uint32_t *rsetQuad = nullptr, *masks = nullptr, *csrPtr = nullptr, *csrIdx = nullptr, *roots = nullptr;
+    uint32_t* order = nullptr;  // workgroup -> graph
     uint4* rows = nullptr;
     int32_t* levels = nullptr;
     hipStream_t stream = nullptr;
@@ -185,32 +188,60 @@ blest_status blest_graph_batch_create(const blest_csc* const* graphs, uint32_t c
             gb.device = device;
             BLEST_HIP(hipSetDevice(device));
             std::vector<dev::BatchGraph> info(count);
-            std::vector<uint32_t> rq, masks, cpAll, ciAll;
-            std::vector<uint32_t> rowWords;
-            uint64_t lv = 0;
-            for (uint32_t i = 0; i < count; ++i) {
+            // Per-graph BVSS (variable layout) built in parallel, then concatenated.
+            std::vector<BVSS> parts(count);
+            std::vector<int> bad(count, 0);
+            for (uint32_t i = 0; i < count; ++i)
                 if (!graphs[i]) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null graph in batch");
+            parallel_dynamic(count, hw_threads(), 1, [&](uint64_t i, unsigned) {
                 const CSC& g = csc_of(graphs[i]);
-                if (g.n == 0 || g.n > 131072) throw Error(BLEST_ERR_UNSUPPORTED, "batched graphs need 1 <= n <= 131072");
-                if (!csc_is_symmetric(g, 1)) throw Error(BLEST_ERR_UNSUPPORTED, "batched graphs must be symmetric");
-                BVSS b = bvss_build(g, 8, 4, false, 1);
+                if (g.n == 0 || g.n > 131072) bad[i] = 1;
+                else if (!csc_is_symmetric(g, 1)) bad[i] = 2;
+                else parts[i] = bvss_build(g, 8, 4, false, 1);
+            });
+            for (uint32_t i = 0; i < count; ++i) {
+                if (bad[i] == 1) throw Error(BLEST_ERR_UNSUPPORTED, "batched graphs need 1 <= n <= 131072");
+                if (bad[i] == 2) throw Error(BLEST_ERR_UNSUPPORTED, "batched graphs must be symmetric");
+            }
+            uint64_t lv = 0, nr = 0, nq = 0, ncp = 0, nci = 0;
+            for (uint32_t i = 0; i < count; ++i) {
+                const CSC& g = csc_of(graphs[i]);
+                const BVSS& b = parts[i];
                 dev::BatchGraph& G = info[i];
                 G.n = g.n;
                 G.R = b.noRealSliceSets;
                 G.levelOff = lv;
-                G.rsetOff = rq.size();
-                G.quadOff = masks.size();
-                G.csrPtrOff = cpAll.size();
-                G.csrIdxOff = ciAll.size();
-                for (uint32_t r = 0; r <= b.noRealSliceSets; ++r) rq.push_back(uint32_t(b.sliceSetPtrs[b.realPtrs[r]] / 4));
-                masks.insert(masks.end(), b.masks.begin(), b.masks.end());
-                rowWords.insert(rowWords.end(), b.rowIds.begin(), b.rowIds.end());
-                cpAll.insert(cpAll.end(), g.colPtrs.begin(), g.colPtrs.end());
-                ciAll.insert(ciAll.end(), g.rows.begin(), g.rows.end());
+                G.rsetOff = nr;
+                G.quadOff = nq;
+                G.csrPtrOff = ncp;
+                G.csrIdxOff = nci;
+                lv += g.n;
+                nr += uint64_t(b.noRealSliceSets) + 1;
+                nq += b.masks.size();
+                ncp += g.colPtrs.size();
+                nci += g.rows.size();
                 gb.sizes.push_back(g.n);
                 gb.maxWords = std::max(gb.maxWords, (g.n + 63) / 64);
-                lv += g.n;
             }
+            std::vector<uint32_t> rq(nr), masks(nq), rowWords(nq * 4), cpAll(ncp), ciAll(nci);
+            parallel_dynamic(count, hw_threads(), 16, [&](uint64_t i, unsigned) {
+                const CSC& g = csc_of(graphs[i]);
+                const BVSS& b = parts[i];
+                const dev::BatchGraph& G = info[i];
+                for (uint32_t r = 0; r <= b.noRealSliceSets; ++r)
+                    rq[G.rsetOff + r] = uint32_t(b.sliceSetPtrs[b.realPtrs[r]] / 4);
+                std::copy(b.masks.begin(), b.masks.end(), masks.begin() + G.quadOff);
+                std::copy(b.rowIds.begin(), b.rowIds.end(), rowWords.begin() + G.quadOff * 4);
+                std::copy(g.colPtrs.begin(), g.colPtrs.end(), cpAll.begin() + G.csrPtrOff);
+                std::copy(g.rows.begin(), g.rows.end(), ciAll.begin() + G.csrIdxOff);
+            });
+            parts.clear();
+            // Workgroups start roughly in index order: largest graphs first, so the
+            // small ones fill in behind them (longest-processing-time order).
+            std::vector<uint32_t> order(count);
+            for (uint32_t i = 0; i < count; ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return info[x].n > info[y].n; });
+            gb.order = gb.upload(order);
             gb.count = count;
             gb.totalN = lv;
             gb.graphs = gb.upload(info);
@@ -250,6 +281,7 @@ blest_status blest_graph_batch_bfs(blest_graph_batch* h, const uint32_t* roots, 
         a.csrPtr = gb.csrPtr;
         a.csrIdx = gb.csrIdx;
         a.roots = gb.roots;
+        a.order = gb.order;
         a.levels = gb.levels;
         a.maxWords = gb.maxWords;
         a.alpha = 15.0;
@@ -259,7 +291,7 @@ blest_status blest_graph_batch_bfs(blest_graph_batch* h, const uint32_t* roots, 
         BLEST_HIP(hipGetLastError());
         BLEST_HIP(hipEventRecord(gb.ev1, gb.stream));
         if (levels)
-            BLEST_HIP(hipMemcpyAsync(levels, gb.levels, 4 * gb.totalN, hipMemcpyDeviceToHost, gb.stream));
+            BLEST_HIP(hipMemcpyAsync(levels, gb.levels, 4 * gb.totalN, hipMemcpyDefault, gb.stream));
         BLEST_HIP(hipStreamSynchronize(gb.stream));
         float ms = 0;
         BLEST_HIP(hipEventElapsedTime(&ms, gb.ev0, gb.ev1));

@@ -101,10 +101,18 @@ __global__ void __launch_bounds__(256) cc_hook_kernel(GraphView g, uint32_t* p) 
     }
 }
 
+// After hooking the roots are final.  Compression must store roots only: a
+// path-halving store (an ancestor that is not the root) racing with another thread's
+// final store could overwrite it.  Every value a thread can read here (an old parent or
+// a root) is an ancestor, so the read-only walk always ends at the root.
 __global__ void cc_compress_kernel(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t r = find_root(p, i);
-        __hip_atomic_store(p + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t x = i, par = ld_parent(p, x);
+        while (par != x) {
+            x = par;
+            par = ld_parent(p, x);
+        }
+        if (x != i) __hip_atomic_store(p + i, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -32,6 +32,7 @@ struct alignas(256) Ctrl {
     unsigned visited;
     unsigned iterations;
     unsigned levelBase;           // first level of the running replayed chunk (graph mode)
+    unsigned histTop;             // multi-source: last level whose frontier words are complete
     unsigned listCount[3];        // frontier list length (vsets) per slot
     unsigned newCount[3];         // vertices (or vertex-root pairs) discovered per slot
     unsigned long long frontEdges[3];  // degree sum of the discovered vertices per slot

@@ -1,6 +1,8 @@
 // Host-side handles of the HBM-resident graph and of per-query device state.
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <functional>
@@ -124,6 +126,8 @@ struct Query {
     // multi-source state, allocated on first use: words per vertex x n
     uint32_t mbfsWords = 0;
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
+    unsigned long long* mHist = nullptr;  // frontier words of levels 0..mHistLevels
+    uint32_t mHistLevels = 0;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries
     LevelGraph graphs[kGraphCount];
@@ -139,6 +143,14 @@ struct Query {
         allocations.push_back(p);
         return static_cast<T*>(p);
     }
+    void release(void* p) {
+        if (!p) return;
+        auto it = std::find(allocations.begin(), allocations.end(), p);
+        if (it == allocations.end()) return;
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(p);
+        allocations.erase(it);
+    }
     ~Query();
 };
 
@@ -152,6 +164,13 @@ void run_levels(Query& q, const std::function<void(unsigned)>& level, const std:
                 uint64_t key = 0);
 uint64_t launch_key(double alpha, double beta, int direction, uint32_t extra);
 uint32_t map_source(const DeviceGraph& g, uint32_t src);
+// Levels of multi-source frontier words kept for the final level expansion: up to 16,
+// within 32 GiB of history per query.
+inline uint32_t mbfs_history_levels(uint32_t n, uint32_t nw) {
+    const uint64_t perLevel = uint64_t(n) * nw * 8;
+    const uint64_t fit = (uint64_t(32) << 30) / std::max<uint64_t>(perLevel, 1);
+    return uint32_t(std::min<uint64_t>(16, std::max<uint64_t>(fit, 2) - 1));
+}
 void bfs_launch(Query& q, uint32_t srcReordered);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);
 

@@ -25,8 +25,10 @@ namespace dev {
 struct MbfsArgs {
     int32_t* levels;
     unsigned long long* visited;
-    unsigned long long* front0;
+    unsigned long long* front0;  // frontier words of levels past the history, by parity
     unsigned long long* front1;
+    unsigned long long* hist;    // frontier words of levels 0..histLevels (n * NW each)
+    uint32_t histLevels;
     unsigned long long* next;
     uint32_t* list0;
     uint32_t* list1;
@@ -80,6 +82,14 @@ __device__ __forceinline__ Bits<NW> root_mask(uint32_t k) {
     return m;
 }
 
+// Frontier words of level L: the history for L <= histLevels (kept so that the final
+// kernel can derive every (root, vertex) level at once), else a parity pair.
+template <int NW>
+__device__ __forceinline__ unsigned long long* mfront(const MbfsArgs& a, uint32_t n, unsigned L) {
+    if (L <= a.histLevels) return a.hist + size_t(L) * n * NW;
+    return (L & 1) ? a.front1 : a.front0;
+}
+
 // Activity word of vertex word w: bit l = some root has vertex 64w+l in frontier f.
 template <int NW>
 __device__ __forceinline__ unsigned long long activity(const GraphView& g, const unsigned long long* f, uint32_t w) {
@@ -118,16 +128,20 @@ __device__ __forceinline__ unsigned segment_list(const GraphView& g, const SegVi
 template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_init_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+    unsigned long long* f0 = a.hist;  // level 0
     for (size_t i = tid; i < size_t(g.n) * NW; i += nthreads) {
         a.visited[i] = 0ull;
-        a.front1[i] = 0ull;
+        f0[i] = 0ull;
         a.next[i] = 0ull;
     }
     for (uint32_t s = tid; s < a.sv.segs; s += nthreads) {
         a.sc.news[a.sv.segs + s] = 0;
         a.sc.edges[a.sv.segs + s] = 0;
     }
-    if (tid == 0) level_state_reset(a.ctrl);
+    if (tid == 0) {
+        level_state_reset(a.ctrl);
+        a.ctrl->histTop = 0;
+    }
 }
 
 // Roots set their bits and level 0, and count into their segment's slot-1 counters.
@@ -138,7 +152,7 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
     const uint32_t v = a.srcs[j];
     const unsigned long long bit = 1ull << (j & 63);
     atomicOr(&a.visited[size_t(v) * NW + j / 64], bit);
-    atomicOr(&a.front1[size_t(v) * NW + j / 64], bit);
+    atomicOr(&a.hist[size_t(v) * NW + j / 64], bit);
     a.levels[size_t(j) * g.n + v] = 0;
     const uint32_t seg = (v >> 6) / a.sv.wps;
     atomicAdd(&a.sc.news[a.sv.segs + seg], 1u);
@@ -148,7 +162,7 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
 template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphView g, MbfsArgs a) {
     __shared__ unsigned tmp[16];
-    const unsigned cnt = segment_list<NW>(g, a.sv, a.front1, a.list1, tmp);
+    const unsigned cnt = segment_list<NW>(g, a.sv, a.hist, a.list1, tmp);
     if (threadIdx.x == 0) a.sc.count[a.sv.segs + blockIdx.x] = cnt;
 }
 
@@ -170,8 +184,9 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
     bool bottomUp = false;
     if (!level_begin(c, a.sc, segs, L, dp, bottomUp)) return;
     const unsigned lane = lane_id();
-    const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
-    unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    const unsigned long long* fin = mfront<NW>(a, g.n, L - 1);
+    unsigned long long* fout = mfront<NW>(a, g.n, L);
+    const bool writeLevels = L > a.histLevels;
     const unsigned sIn = L % 3, sOut = (L + 1) % 3;
     if (!bottomUp) {
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
@@ -270,9 +285,10 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
 #pragma unroll
                 for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
                 vis.store(a.visited, v);
-                for (int i = 0; i < NW; ++i)
-                    for (unsigned long long b = nw.w[i]; b; b &= b - 1)
-                        a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                if (writeLevels)
+                    for (int i = 0; i < NW; ++i)
+                        for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                            a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
                 news += nw.popc();
                 edges += (unsigned long long)(p1 - p0) * nw.popc();
             }
@@ -295,10 +311,12 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
     const bool bottomUp = c->state[L & 1].dir != 0;
-    unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
+    unsigned long long* fout = mfront<NW>(a, g.n, L);
+    const bool writeLevels = L > a.histLevels;
     uint32_t* lout = (L & 1) ? a.list0 : a.list1;
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->histTop = L;  // read only after this kernel
     if (bottomUp) {
         const unsigned cnt = segment_list<NW>(g, a.sv, fout, lout, tmp);
         if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = cnt;
@@ -324,9 +342,10 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
                 for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
                 vis.store(a.visited, v);
                 const unsigned pc = nw.popc();
-                for (int i = 0; i < NW; ++i)
-                    for (unsigned long long b = nw.w[i]; b; b &= b - 1)
-                        a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                if (writeLevels)
+                    for (int i = 0; i < NW; ++i)
+                        for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                            a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
                 news += pc;
                 if (g.csrPtr) edges += (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]) * pc;
             }
@@ -344,15 +363,43 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     }
 }
 
-// Unreached (root, vertex) pairs get -1; root-major rows, so stores are coalesced.
+// Every (root, vertex) level at once, thread per vertex: the level of root j is the
+// history level whose frontier word holds bit j (kept in LDS as a byte, [root][thread]
+// so that both the scatter and the read-out are conflict-free); pairs found past the
+// history were written when found; the rest are unreached (-1).  Root-major rows, so
+// for each root the 256 threads store 256 consecutive levels.  Idempotent: enqueued
+// ahead of each completion poll, it reads only levels whose words are complete.
+constexpr unsigned kFinalBlock = 256;
+
 template <int NW>
-__global__ void __launch_bounds__(kProducerBlock) mbfs_final_kernel(GraphView g, MbfsArgs a) {
-    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
-    for (uint32_t v = tid; v < g.n; v += nthreads) {
-        Bits<NW> vis;
-        vis.load(a.visited, v);
-        for (uint32_t j = 0; j < a.k; ++j)
-            if (!((vis.w[j / 64] >> (j & 63)) & 1ull)) a.levels[size_t(j) * g.n + v] = -1;
+__global__ void __launch_bounds__(kFinalBlock) mbfs_final_kernel(GraphView g, MbfsArgs a) {
+    __shared__ uint8_t lv[64 * kFinalBlock];
+    const unsigned top = min(__hip_atomic_load(&a.ctrl->histTop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                             a.histLevels);
+    const unsigned t = threadIdx.x;
+    for (uint32_t v0 = blockIdx.x * kFinalBlock; v0 < g.n; v0 += gridDim.x * kFinalBlock) {
+        const uint32_t v = v0 + t;
+        const bool live = v < g.n;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            if (64 * i >= int(a.k)) break;
+            for (int j = 0; j < 64; ++j) lv[j * kFinalBlock + t] = 0xFF;
+            unsigned long long vis = 0;
+            if (live) {
+                vis = a.visited[size_t(v) * NW + i];
+                for (unsigned L = 0; L <= top; ++L)
+                    for (unsigned long long x = a.hist[(size_t(L) * g.n + v) * NW + i]; x; x &= x - 1)
+                        lv[__builtin_ctzll(x) * kFinalBlock + t] = uint8_t(L);
+            }
+            const unsigned jn = min(64u, a.k - 64u * i);
+            for (unsigned j = 0; j < jn; ++j) {
+                const unsigned l = lv[j * kFinalBlock + t];
+                if (!live) continue;
+                int32_t* out = a.levels + size_t(64 * i + j) * g.n + v;
+                if (l != 0xFF) *out = int32_t(l);
+                else if (!((vis >> j) & 1ull)) *out = -1;
+            }
+        }
     }
 }
 
@@ -417,13 +464,19 @@ __global__ void __launch_bounds__(256) root_stats_kernel(RootStats* st, const in
 }  // namespace dev
 
 static void mbfs_ensure_state(Query& q, uint32_t nw) {
+    // (declared in dgraph.hpp: the history depth also sizes the query footprint)
     DeviceGraph& g = *q.g;
     if (q.mbfsWords >= nw) return;
+    for (void* p : {static_cast<void*>(q.mVis), static_cast<void*>(q.mFront0), static_cast<void*>(q.mFront1),
+                    static_cast<void*>(q.mNext), static_cast<void*>(q.mHist)})
+        q.release(p);
     const size_t words = size_t(g.n) * nw;
     q.mVis = q.alloc<unsigned long long>(words);
     q.mFront0 = q.alloc<unsigned long long>(words);
     q.mFront1 = q.alloc<unsigned long long>(words);
     q.mNext = q.alloc<unsigned long long>(words);
+    q.mHistLevels = mbfs_history_levels(g.n, nw);
+    q.mHist = q.alloc<unsigned long long>(words * (q.mHistLevels + 1));
     if (!q.srcs) q.srcs = q.alloc<uint32_t>(128);
     if (!q.rootStats) q.rootStats = q.alloc<dev::RootStats>(128);
     q.mbfsWords = nw;
@@ -445,7 +498,10 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
             hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
         },
-        [&] { hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, grid, block, 0, q.stream, gv, a); },
+        [&] {
+            const uint32_t fb = std::min<uint32_t>((g.n + dev::kFinalBlock - 1) / dev::kFinalBlock, uint32_t(g.cus) * 8);
+            hipLaunchKernelGGL(dev::mbfs_final_kernel<NW>, dim3(fb), dim3(dev::kFinalBlock), 0, q.stream, gv, a);
+        },
         &q.graphs[NW == 1 ? kGraphMbfs1 : kGraphMbfs2],
         [&](unsigned k) {
             hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
@@ -472,6 +528,8 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.visited = q.mVis;
     a.front0 = q.mFront0;
     a.front1 = q.mFront1;
+    a.hist = q.mHist;
+    a.histLevels = q.mHistLevels;
     a.next = q.mNext;
     a.list0 = q.list0;
     a.list1 = q.list1;
