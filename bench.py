@@ -328,6 +328,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--roofline-roots", type=int, default=-1,
+                    help="roots replayed for the byte model (-1: all up to 2^23 vertices, else 8; 0: all)")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--alpha", type=float, default=0.0)
@@ -414,7 +416,11 @@ def main():
 
     roofline = None
     cpu = None
+    if a.roofline_roots < 0:
+        a.roofline_roots = 0 if prep.csc.n <= (1 << 23) else 8
     if not a.no_roofline or not a.no_cpu_baseline:
+        import ctypes as C
+
         import oracle
         bv = prep.bvss.arrays()
         cp, rows = prep.csc.arrays()  # symmetric: the CSR of the bottom-up pass is the CSC
@@ -423,24 +429,47 @@ def main():
         # Algorithmic bytes per launch: the reference formulation's per-level bytes on the
         # direction schedule the launch actually ran (one replay per root; deterministic).
         t = time.time()
-        per_root = {}
-        for r in roots:
+        per_root, per_level = {}, {}
+        replay_roots = roots if a.roofline_roots <= 0 else roots[:a.roofline_roots]
+        for r in replay_roots:
             mask = next(s[3] for s in samples if s[0] == r)
             sched = [(mask >> i) & 1 for i in range(64)]
             _, _, levels = oracle.bvss_bfs(prep.csc.n, bv, cp, rows, int(prep.inv_perm[r]), schedule=sched,
                                            threads=threads)
             per_root[r] = sum(l["bytes"] for l in levels)
-        total_bytes = sum(per_root[s[0]] for s in samples)
-        total_s = sum(s[1] for s in samples) * 1e-3
+            per_level[r] = levels
+        used = [s for s in samples if s[0] in per_root]
+        total_bytes = sum(per_root[s[0]] for s in used)
+        total_s = sum(s[1] for s in used) * 1e-3
         achieved = total_bytes / total_s / 1e9
+        # per direction: each replayed root traced once (a HIP event around every kernel)
+        tr = (B._lib.LevelTrace * 4096)()
+        cnt = C.c_uint32()
+        st = B._lib.BfsStats()
+        by_dir = {0: [0, 0.0], 1: [0, 0.0]}  # bytes, seconds
+        for r in replay_roots:
+            B.check(lib.blest_bfs_trace(q, r, None, C.byref(st), tr, 4096, C.byref(cnt)))
+            for i, lv in enumerate(per_level[r]):
+                if i >= cnt.value:
+                    break
+                d = int(tr[i].bottom_up)
+                by_dir[d][0] += lv["bytes"]
+                by_dir[d][1] += (tr[i].expand_ms + tr[i].update_ms) * 1e-3
+        directions = {}
+        for d, name in ((0, "top_down"), (1, "bottom_up")):
+            if by_dir[d][1] > 0:
+                ach = by_dir[d][0] / by_dir[d][1] / 1e9
+                directions[name] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                                    "ms_per_launch": round(by_dir[d][1] * 1e3 / len(replay_roots), 4)}
         traffic = None
         tf = ROOT / "profiles" / f"bfs_kernel_{a.workload}_traffic.json"
         if tf.exists():
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "bfs_kernel", "algorithmic_bytes_per_launch": int(total_bytes / len(samples)),
-                    "mean_launch_ms": round(mean_ms, 4)}
+                    "kernel": "bfs_kernel", "algorithmic_bytes_per_launch": int(total_bytes / len(used)),
+                    "mean_launch_ms": round(mean_ms, 4), "replayed_roots": len(replay_roots),
+                    "by_direction": directions}
         log(f"[bench] byte model replayed in {time.time() - t:.1f}s: {achieved:.1f} GB/s")
     if not a.no_cpu_baseline:
         # Reference CPU path: level-synchronous BVSS BFS with the reference's switching

@@ -57,6 +57,11 @@ class BfsStats(C.Structure):
                 ("edges_traversed_component", C.c_uint64), ("bottom_up_mask", C.c_uint64)]
 
 
+class LevelTrace(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("bottom_up", C.c_uint32), ("expand_ms", C.c_float),
+                ("update_ms", C.c_float)]
+
+
 class MbfsOpts(C.Structure):
     _fields_ = [("algo", C.c_int), ("batch", C.c_uint32)]
 
@@ -127,6 +132,8 @@ PROTOTYPES = {
     "blest_bfs_async": (C.c_int, [P, C.c_uint32, P]),
     "blest_query_sync": (C.c_int, [P, C.POINTER(BfsStats)]),
     "blest_query_levels_device": (P, [P]),
+    "blest_bfs_trace": (C.c_int, [P, C.c_uint32, I32P, C.POINTER(BfsStats), C.POINTER(LevelTrace), C.c_uint32,
+                                  U32P]),
     "blest_mbfs": (C.c_int, [P, U32P, C.c_uint32, I32P, C.POINTER(MbfsOpts), C.POINTER(BfsStats)]),
     "blest_cc": (C.c_int, [P, C.c_int, U32P, C.POINTER(CcStats)]),
     "blest_device_graph_set_thresholds": (C.c_int, [P, C.c_double, C.c_double, C.c_int]),

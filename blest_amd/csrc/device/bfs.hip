@@ -270,17 +270,10 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
         }
     }
     cnt = wave_sum_u32(cnt);
-    // pass 2: one block scan places each wave's entries; lane per word writes them
+    // pass 2: one block scan places each wave's entries; the wave writes them balanced
     unsigned running;
     uint32_t* out = lout + a.sv.segBase[blockIdx.x] + wave_offset_in_block(cnt, tmp, running);
-    for (uint32_t c0 = first; c0 < last; c0 += kWave) {
-        const uint32_t cw = c0 + lane;
-        const unsigned long long mine = (cw < last) ? fout[cw] : 0ull;
-        const unsigned k = word_vsets(g.rsetUnit, cw, mine);
-        const unsigned incl = wave_inclusive_scan(k);
-        write_word_vsets(g.rsetUnit, cw, mine, out + incl - k);
-        out += __shfl(incl, kWave - 1, kWave);
-    }
+    wave_write_vsets(g.rsetUnit, first, last, [&](uint32_t w) { return fout[w]; }, out);
     if (!bottomUp) {
         news = block_sum_u64(news, tmp64);
         edges = block_sum_u64(edges, tmp64);
@@ -313,7 +306,7 @@ __global__ void unpermute_levels_kernel(int32_t* out, const int32_t* in, const u
 
 }  // namespace dev
 
-void bfs_launch(Query& q, uint32_t src) {
+void bfs_launch(Query& q, uint32_t src, bool trace) {
     DeviceGraph& g = *q.g;
     dev::BfsArgs a;
     a.levels = q.levels;
@@ -337,6 +330,32 @@ void bfs_launch(Query& q, uint32_t src) {
     const dim3 grid = level_grid(g), block(dev::kProducerBlock);
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
     hipLaunchKernelGGL(dev::bfs_init_kernel, grid, block, 0, q.stream, gv, a);
+    if (trace) {
+        size_t next = 0;
+        auto mark = [&] {
+            if (next == q.traceEvents.size()) {
+                hipEvent_t e;
+                BLEST_HIP(hipEventCreate(&e));
+                q.traceEvents.push_back(e);
+            }
+            BLEST_HIP(hipEventRecord(q.traceEvents[next++], q.stream));
+        };
+        run_levels(
+            q,
+            [&](unsigned L) {
+                mark();
+                hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, a, L);
+                mark();
+                hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, a, L);
+                mark();
+            },
+            [&] { hipLaunchKernelGGL(dev::bfs_final_kernel, grid, block, 0, q.stream, gv, a); });
+        BLEST_HIP(hipGetLastError());
+        BLEST_HIP(hipEventRecord(q.ev1, q.stream));
+        q.lastSource = src;
+        q.pending = true;
+        return;
+    }
     run_levels(
         q,
         [&](unsigned L) {
@@ -423,5 +442,33 @@ blest_status blest_query_sync(blest_query* qh, blest_bfs_stats* st) {
 }
 
 void* blest_query_levels_device(blest_query* qh) { return qh ? qh->q.levels : nullptr; }
+
+blest_status blest_bfs_trace(blest_query* qh, uint32_t src, int32_t* levels, blest_bfs_stats* st,
+                             blest_level_trace* trace, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!qh || (cap && !trace)) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        Query& q = qh->q;
+        std::lock_guard<std::mutex> lk(q.mu);
+        BLEST_HIP(hipSetDevice(q.g->device));
+        bfs_launch(q, map_source(*q.g, src), true);
+        blest_bfs_stats s{};
+        bfs_finish(q, levels, &s);
+        if (st) *st = s;
+        // events per level L = 1, 2, ...: before expand(L), after it, after update(L)
+        const uint32_t nl = std::min<uint32_t>(s.total_levels, uint32_t(q.traceEvents.size() / 3));
+        const uint32_t m = std::min(cap, nl);
+        for (uint32_t i = 0; i < m; ++i) {
+            const unsigned L = i + 1;
+            float e = 0, u = 0;
+            BLEST_HIP(hipEventElapsedTime(&e, q.traceEvents[3 * i], q.traceEvents[3 * i + 1]));
+            BLEST_HIP(hipEventElapsedTime(&u, q.traceEvents[3 * i + 1], q.traceEvents[3 * i + 2]));
+            trace[i].level = L;
+            trace[i].bottom_up = L <= 64 ? uint32_t((s.bottom_up_mask >> (L - 1)) & 1ull) : 0u;
+            trace[i].expand_ms = e;
+            trace[i].update_ms = u;
+        }
+        if (count) *count = m;
+    });
+}
 
 }  // extern "C"

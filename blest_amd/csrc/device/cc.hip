@@ -252,7 +252,13 @@ __global__ void __launch_bounds__(kProducerBlock) cc_lp_update_kernel(GraphView 
         const unsigned cnt = word_vsets(g.rsetUnit, w, mk);
         unsigned tot;
         const unsigned ex = block_exclusive_scan(cnt, tmp, tot);
-        write_word_vsets(g.rsetUnit, w, mk, out + running + ex);
+        // each wave writes its 64 words' vsets, balanced over its lanes
+        const unsigned waveStart = __shfl(ex, 0, kWave);
+        const uint32_t w0 = r0 + (threadIdx.x / kWave) * kWave;
+        const uint32_t w1 = min(we, w0 + kWave);
+        if (w0 < w1)
+            wave_write_vsets(g.rsetUnit, w0, w1, [&](uint32_t x) { return __shfl(mk, x - w0, kWave); },
+                             out + running + waveStart);
         running += tot;
     }
     if (threadIdx.x == 0) a.sc.count[size_t((L + 1) % 3) * a.sv.segs + blockIdx.x] = running;

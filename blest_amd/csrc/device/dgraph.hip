@@ -16,6 +16,7 @@ Query::~Query() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto& lg : graphs)
         if (lg.exec) (void)hipGraphExecDestroy(lg.exec);
+    for (hipEvent_t e : traceEvents) (void)hipEventDestroy(e);
     for (void* p : allocations) (void)hipFree(p);
     if (hostDone) (void)hipHostFree(hostDone);
     if (ev0) (void)hipEventDestroy(ev0);

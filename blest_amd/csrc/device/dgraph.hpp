@@ -131,6 +131,7 @@ struct Query {
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries
     LevelGraph graphs[kGraphCount];
+    std::vector<hipEvent_t> traceEvents;  // one per traced kernel boundary (created on demand)
     std::vector<void*> allocations;
     std::mutex mu;
     uint32_t lastSource = 0;
@@ -171,7 +172,8 @@ inline uint32_t mbfs_history_levels(uint32_t n, uint32_t nw) {
     const uint64_t fit = (uint64_t(32) << 30) / std::max<uint64_t>(perLevel, 1);
     return uint32_t(std::min<uint64_t>(16, std::max<uint64_t>(fit, 2) - 1));
 }
-void bfs_launch(Query& q, uint32_t srcReordered);
+// trace (nullable): record an event after the init kernel and after every level kernel.
+void bfs_launch(Query& q, uint32_t srcReordered, bool trace = false);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);
 
 }  // namespace blest

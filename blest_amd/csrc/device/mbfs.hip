@@ -109,7 +109,17 @@ __device__ __forceinline__ unsigned write_segment_list(const GraphView& g, const
                                                        uint32_t* list, unsigned* tmp) {
     unsigned total;
     uint32_t* out = list + sv.segBase[blockIdx.x] + wave_offset_in_block(cnt, tmp, total);
-    for (uint32_t w = first; w < last; ++w) out += wave_write_word_vsets(g.rsetUnit, w, activity<NW>(g, f, w), out);
+    // activity needs the whole wave, so the words' activity is gathered first, 64 words
+    // at a time (lane = word), then written balanced
+    for (uint32_t c0 = first; c0 < last; c0 += kWave) {
+        unsigned long long act = 0;
+        const uint32_t c1 = min(last, c0 + kWave);
+        for (uint32_t w = c0; w < c1; ++w) {
+            const unsigned long long x = activity<NW>(g, f, w);
+            if (lane_id() == w - c0) act = x;
+        }
+        out += wave_write_vsets(g.rsetUnit, c0, c1, [&](uint32_t w) { return __shfl(act, w - c0, kWave); }, out);
+    }
     return total;
 }
 

@@ -128,6 +128,41 @@ __device__ __forceinline__ void write_word_vsets(const uint32_t* rsetUnit, uint3
             for (uint32_t u = rsetUnit[w * 8 + b]; u < rsetUnit[w * 8 + b + 1]; ++u) *out++ = u;
 }
 
+// Writes, at out, the vsets of the rsets active in words [first, last) (in word and
+// rset order; bitsOf(w) gives word w's frontier bits) and returns their number.  Lane
+// = one of the 64 rsets of 8 consecutive words; each step's entries are then written
+// by all 64 lanes (owner search), so an rset with thousands of vsets (a hub's) costs
+// the wave as many stores as any other 64 entries.  The whole wave must call.
+template <class BitsOf>
+__device__ __forceinline__ unsigned wave_write_vsets(const uint32_t* rsetUnit, uint32_t first, uint32_t last,
+                                                     BitsOf bitsOf, uint32_t* out) {
+    const unsigned lane = lane_id();
+    unsigned written = 0;
+    for (uint32_t c0 = first; c0 < last; c0 += kWave / 8) {
+        const uint32_t w = c0 + lane / 8, b = lane % 8;
+        uint32_t start = 0, cnt = 0;
+        const unsigned long long bits = bitsOf(min(w, last - 1));  // every lane calls (bitsOf may shuffle)
+        if (w < last) {
+            if ((bits >> (8 * b)) & 0xFFull) {
+                start = rsetUnit[w * 8 + b];
+                cnt = rsetUnit[w * 8 + b + 1] - start;
+            }
+        }
+        const unsigned incl = wave_inclusive_scan(cnt);
+        const unsigned excl = incl - cnt;
+        const unsigned total = __shfl(incl, kWave - 1, kWave);
+        for (unsigned t0 = 0; t0 < total; t0 += kWave) {
+            const unsigned t = t0 + lane;
+            const unsigned owner = wave_owner(excl, min(t, total - 1));
+            const uint32_t os = __shfl(start, owner, kWave);
+            const unsigned oe = __shfl(excl, owner, kWave);
+            if (t < total) out[written + t] = os + (t - oe);
+        }
+        written += total;
+    }
+    return written;
+}
+
 // Word range [first, last) of wave `wib` when a segment [ws, we) is cut into
 // contiguous runs, one per wave of the workgroup.
 __device__ __forceinline__ void wave_words(uint32_t ws, uint32_t we, unsigned wib, unsigned wpb, uint32_t& first,

@@ -8,6 +8,7 @@ Workloads (sizes as named; every seed fixed):
   rmat16_natural   R-MAT scale 16, edge factor 16, natural order, 16 roots
   rmat22_gorder    R-MAT scale 22, edge factor 16, Gorder+Jaccard window 65,536, 64 roots
   grid4096_rcm     4096x4096 grid, keep 0.9, 1% shortcuts, RCM, 64 roots
+  rmat26           R-MAT scale 26, edge factor 16, natural order, 64 roots
   rmat26_mbfs      R-MAT scale 26, edge factor 16, natural order, 1,024 roots (MBFS)
 """
 from __future__ import annotations
@@ -54,6 +55,7 @@ WORKLOADS = {
     "grid4096_rcm": Workload("grid4096_rcm", "grid", side=4096, reorder=Reorder.RCM, roots=64),
     "grid4096_plain_rcm": Workload("grid4096_plain_rcm", "grid", side=4096, keep=1.0, shortcut_frac=0.0,
                                    reorder=Reorder.RCM, roots=64),
+    "rmat26": Workload("rmat26", "rmat", scale=26, reorder=Reorder.NATURAL, roots=64),
     "rmat26_mbfs": Workload("rmat26_mbfs", "rmat", scale=26, reorder=Reorder.NATURAL, roots=1024),
 }
 

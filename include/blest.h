@@ -246,6 +246,17 @@ void* blest_query_levels_device(blest_query* q);
 /* Multi-source BFS (MBFS/MBFSKernel.cuh:21-53; the reference kernel file is
  * empty): k independent traversals in one launch, k <= max_sources of q.
  * levels[k*n] (row r = root r). */
+/* Per-level timing (tracing; no reference counterpart beyond its commented-out
+ * per-level clock stamps, BFS/BVSSBFSKernel.cuh:56-58): a single-source BFS that
+ * records a HIP event after every kernel.  Level L = 1..total_levels - 1 expands
+ * frontier L-1 (bottom_up = direction taken) and folds the result into frontier L. */
+typedef struct blest_level_trace {
+    uint32_t level, bottom_up;
+    float expand_ms, update_ms;
+} blest_level_trace;
+blest_status blest_bfs_trace(blest_query* q, uint32_t src, int32_t* levels, blest_bfs_stats* st,
+                             blest_level_trace* trace, uint32_t cap, uint32_t* count);
+
 typedef struct blest_mbfs_opts {
     int algo;            /* 0 = bit-parallel VALU, 1 = i8 MFMA block product */
     uint32_t batch;      /* roots per launch: 64 or 128 */

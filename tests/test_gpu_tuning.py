@@ -46,3 +46,28 @@ def test_set_thresholds_rejects(blest):
         dg.set_thresholds(15, 18, 0)
     with pytest.raises(blest.BlestError):
         dg.set_thresholds(-1, 18, 1)
+
+
+def test_bfs_trace_matches_plain_run(blest, oracle):
+    """The traced run gives the same levels and one timing entry per level."""
+    import ctypes as C
+    print("case bfs trace", flush=True)
+    g = blest.CSC.rmat(13, 16, seed=12)
+    cp, rows = g.arrays()
+    b = blest.BVSS(8, 4).construct_from_csc(g)
+    dg = blest.DeviceGraph(b, g, None)
+    q = blest._Query(dg, 1)
+    lib = blest._lib.load()
+    for src in g.pick_roots(3, seed=4):
+        lv = np.empty(g.n, dtype=np.int32)
+        st = blest._lib.BfsStats()
+        tr = (blest._lib.LevelTrace * 64)()
+        cnt = C.c_uint32()
+        blest._lib.check(lib.blest_bfs_trace(q._h, int(src), lv.ctypes.data_as(blest._lib.I32P), C.byref(st), tr,
+                                             64, C.byref(cnt)))
+        want = oracle.bfs(g.n, cp, rows, int(src))
+        np.testing.assert_array_equal(lv, want)
+        assert cnt.value == st.total_levels
+        for i in range(cnt.value):
+            assert tr[i].level == i + 1 and tr[i].expand_ms >= 0 and tr[i].update_ms >= 0
+            assert tr[i].bottom_up == ((st.bottom_up_mask >> i) & 1)
