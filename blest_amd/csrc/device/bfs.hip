@@ -202,6 +202,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
 __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[16];
+    __shared__ unsigned sEx[kProducerBlock + 1];
+    __shared__ uint32_t sStart[kProducerBlock];
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
@@ -215,28 +217,21 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
     uint32_t first, last;
     wave_words(ws, we, wib, wpb, first, last);
     const bool canBU = g.csrPtr != nullptr;
-    unsigned cnt = 0;
     unsigned long long news = 0, edges = 0;
-    // pass 1: lane per word (one coalesced load per 64 words) folds `next` into
-    // visited / frontier and counts list entries; the new vertices' levels are then
-    // written lane per vertex (coalesced), four words' degree loads in flight at once.
-    for (uint32_t c0 = first; c0 < last; c0 += kWave) {
+    // pass 1 (top-down levels): lane per word (one coalesced load per 64 words) folds
+    // `next` into visited / frontier; the new vertices' levels are then written lane per
+    // vertex (coalesced), four words' degree loads in flight at once.
+    for (uint32_t c0 = first; c0 < last && !bottomUp; c0 += kWave) {
         const uint32_t cw = c0 + lane;
         unsigned long long mine = 0;
         if (cw < last) {
-            if (bottomUp) {
-                mine = fout[cw];
-            } else {
-                mine = a.next[cw];
-                if (mine) {
-                    a.next[cw] = 0ull;
-                    a.visited[cw] |= mine;
-                }
-                fout[cw] = mine;
+            mine = a.next[cw];
+            if (mine) {
+                a.next[cw] = 0ull;
+                a.visited[cw] |= mine;
             }
+            fout[cw] = mine;
         }
-        cnt += word_vsets(g.rsetUnit, cw, mine);
-        if (bottomUp) continue;
         news += __popcll(mine);
         unsigned long long nz = __ballot(mine != 0);
         while (nz) {
@@ -269,11 +264,11 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
             for (int t = 0; t < 4; ++t) edges += hi[t] - lo[t];
         }
     }
-    cnt = wave_sum_u32(cnt);
-    // pass 2: one block scan places each wave's entries; the wave writes them balanced
-    unsigned running;
-    uint32_t* out = lout + a.sv.segBase[blockIdx.x] + wave_offset_in_block(cnt, tmp, running);
-    wave_write_vsets(g.rsetUnit, first, last, [&](uint32_t w) { return fout[w]; }, out);
+    __syncthreads();  // pass 1's frontier words are read by other waves below
+    // pass 2: the segment's vset list, written by the whole workgroup
+    const unsigned running = block_write_vsets(
+        g.rsetUnit, ws, we, [](uint32_t) {}, [&](uint32_t w) { return fout[w]; },
+        lout + a.sv.segBase[blockIdx.x], sEx, sStart, tmp);
     if (!bottomUp) {
         news = block_sum_u64(news, tmp64);
         edges = block_sum_u64(edges, tmp64);

@@ -100,39 +100,34 @@ __device__ __forceinline__ unsigned long long activity(const GraphView& g, const
     return __ballot(b.any());
 }
 
-// Writes the vsets of the rsets active in frontier f over this wave's words
-// [first, last) at this workgroup's list range, placed by one block scan over the
-// waves' counts.  Returns the segment total.  Must be called by the whole block.
-template <int NW>
-__device__ __forceinline__ unsigned write_segment_list(const GraphView& g, const SegView& sv, uint32_t first,
-                                                       uint32_t last, unsigned cnt, const unsigned long long* f,
-                                                       uint32_t* list, unsigned* tmp) {
-    unsigned total;
-    uint32_t* out = list + sv.segBase[blockIdx.x] + wave_offset_in_block(cnt, tmp, total);
-    // activity needs the whole wave, so the words' activity is gathered first, 64 words
-    // at a time (lane = word), then written balanced
-    for (uint32_t c0 = first; c0 < last; c0 += kWave) {
-        unsigned long long act = 0;
-        const uint32_t c1 = min(last, c0 + kWave);
-        for (uint32_t w = c0; w < c1; ++w) {
-            const unsigned long long x = activity<NW>(g, f, w);
-            if (lane_id() == w - c0) act = x;
-        }
-        out += wave_write_vsets(g.rsetUnit, c0, c1, [&](uint32_t w) { return __shfl(act, w - c0, kWave); }, out);
-    }
-    return total;
-}
+// The vset list of a frontier held as per-vertex words, for this workgroup's segment,
+// written by the whole workgroup (block_write_vsets); each chunk's 64 word activities
+// are first staged in LDS (wave w computes words 8w .. 8w+7).  Returns the count.
+struct ListLds {
+    unsigned tmp[16];
+    unsigned ex[kProducerBlock + 1];
+    uint32_t start[kProducerBlock];
+    unsigned long long act[kProducerBlock / 8];
+};
 
-// The vset list of a frontier held as per-vertex words, for this workgroup's segment.
 template <int NW>
 __device__ __forceinline__ unsigned segment_list(const GraphView& g, const SegView& sv,
-                                                 const unsigned long long* f, uint32_t* list, unsigned* tmp) {
+                                                 const unsigned long long* f, uint32_t* list, ListLds& l) {
     const uint32_t ws = blockIdx.x * sv.wps, we = min(g.W, ws + sv.wps);
-    uint32_t first, last;
-    wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
-    unsigned cnt = 0;
-    for (uint32_t w = first; w < last; ++w) cnt += wave_word_vsets(g.rsetUnit, w, activity<NW>(g, f, w));
-    return write_segment_list<NW>(g, sv, first, last, cnt, f, list, tmp);
+    const unsigned wave = threadIdx.x / kWave, per = blockDim.x / 8 / (blockDim.x / kWave);
+    auto prep = [&](uint32_t c0) {
+        for (unsigned j = 0; j < per; ++j) {
+            const unsigned i = wave * per + j;
+            const uint32_t w = c0 + i;
+            const unsigned long long x = w < we ? activity<NW>(g, f, w) : 0ull;
+            if (lane_id() == 0) l.act[i] = x;
+        }
+        __syncthreads();
+    };
+    // chunks start at ws + k * blockDim/8, so a word's place in its chunk is (w - ws) % (blockDim/8)
+    return block_write_vsets(
+        g.rsetUnit, ws, we, prep, [&](uint32_t w) { return l.act[(w - ws) % (blockDim.x / 8)]; },
+        list + sv.segBase[blockIdx.x], l.ex, l.start, l.tmp);
 }
 
 template <int NW>
@@ -171,8 +166,8 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
 
 template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphView g, MbfsArgs a) {
-    __shared__ unsigned tmp[16];
-    const unsigned cnt = segment_list<NW>(g, a.sv, a.hist, a.list1, tmp);
+    __shared__ ListLds l;
+    const unsigned cnt = segment_list<NW>(g, a.sv, a.hist, a.list1, l);
     if (threadIdx.x == 0) a.sc.count[a.sv.segs + blockIdx.x] = cnt;
 }
 
@@ -316,7 +311,7 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
 template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
-    __shared__ unsigned tmp[16];
+    __shared__ ListLds l;
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
@@ -328,14 +323,13 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     const unsigned sOut = (L + 1) % 3;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->histTop = L;  // read only after this kernel
     if (bottomUp) {
-        const unsigned cnt = segment_list<NW>(g, a.sv, fout, lout, tmp);
+        const unsigned cnt = segment_list<NW>(g, a.sv, fout, lout, l);
         if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = cnt;
         return;
     }
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
     uint32_t first, last;
     wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
-    unsigned cnt = 0;
     unsigned long long news = 0, edges = 0;
     for (uint32_t w = first; w < last; ++w) {
         const uint32_t v = w * 64 + lane_id();
@@ -361,9 +355,9 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
             }
             nw.store(fout, v);
         }
-        cnt += wave_word_vsets(g.rsetUnit, w, __ballot(nw.any()));
     }
-    const unsigned running = write_segment_list<NW>(g, a.sv, first, last, cnt, fout, lout, tmp);
+    __syncthreads();  // the frontier words above are read by other waves below
+    const unsigned running = segment_list<NW>(g, a.sv, fout, lout, l);
     news = block_sum_u64(news, tmp64);
     edges = block_sum_u64(edges, tmp64);
     if (threadIdx.x == 0) {

@@ -163,6 +163,47 @@ __device__ __forceinline__ unsigned wave_write_vsets(const uint32_t* rsetUnit, u
     return written;
 }
 
+// Block-wide form for a producer workgroup's whole segment [ws, we): thread = one rset
+// of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
+// are written by all threads, entry e by thread e % blockDim (coalesced), each thread
+// walking an owner cursor over the chunk's prefix in LDS.  A segment holding hub rsets
+// (hundreds of thousands of vsets) is so spread over the whole workgroup.  prep(c0) runs
+// on every thread before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds
+// blockDim + 1 entries, sStart blockDim.  Returns the segment's entry count.
+template <class Prep, class BitsOf>
+__device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, uint32_t ws, uint32_t we, Prep prep,
+                                                      BitsOf bitsOf, uint32_t* out, unsigned* sEx, uint32_t* sStart,
+                                                      unsigned* tmp) {
+    const unsigned t = threadIdx.x, nb = blockDim.x;
+    unsigned written = 0;
+    for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
+        prep(c0);
+        const uint32_t w = c0 + t / 8, b = t % 8;
+        uint32_t start = 0, cnt = 0;
+        if (w < we) {
+            const unsigned long long bits = bitsOf(w);
+            if ((bits >> (8 * b)) & 0xFFull) {
+                start = rsetUnit[w * 8 + b];
+                cnt = rsetUnit[w * 8 + b + 1] - start;
+            }
+        }
+        unsigned total;
+        const unsigned ex = block_exclusive_scan(cnt, tmp, total);
+        sEx[t] = ex;
+        sStart[t] = start;
+        if (t == 0) sEx[nb] = total;
+        __syncthreads();
+        unsigned cur = 0;
+        for (unsigned e = t; e < total; e += nb) {
+            while (sEx[cur + 1] <= e) ++cur;
+            out[written + e] = sStart[cur] + (e - sEx[cur]);
+        }
+        written += total;
+        __syncthreads();  // the next chunk reuses sEx / sStart
+    }
+    return written;
+}
+
 // Word range [first, last) of wave `wib` when a segment [ws, we) is cut into
 // contiguous runs, one per wave of the workgroup.
 __device__ __forceinline__ void wave_words(uint32_t ws, uint32_t we, unsigned wib, unsigned wpb, uint32_t& first,
