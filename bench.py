@@ -196,14 +196,14 @@ def serve_bench(a) -> dict:
             "rates": rows, "roofline": None, "cpu_baseline": None}
 
 
-def mbfs_bench(a, prep) -> dict:
+def mbfs_bench(a, prep, opts) -> dict:
     """Batched multi-source BFS against plain single-source runs on the same roots:
     aggregate GTEPS = sum of m_c / sum of kernel time; speedup = batched / plain."""
     import ctypes as C
 
     import blest_amd as B
     lib = B._lib.load()
-    dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm)
+    dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
     roots = [int(r) for r in prep.roots]
     if a.mbfs_roots > len(roots):
         # more seeded roots of degree >= 1, picked on the resident (reordered) graph and
@@ -334,6 +334,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
+    ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
     ap.add_argument("--serve-scales", default="", help="serve: comma-separated R-MAT scales (default 18..25)")
     ap.add_argument("--serve-grids", default="", help="serve: comma-separated grid sides (default 2048,4096)")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -369,9 +370,6 @@ def main():
     from blest_amd import workloads as W
 
     prep = W.prepare(a.workload, log=log)
-    if a.mbfs:
-        print(json.dumps(mbfs_bench(a, prep)), flush=True)
-        return
     w = prep.workload
     opts = {}
     if a.block_size:
@@ -382,6 +380,11 @@ def main():
         opts["alpha"] = a.alpha
     if a.beta:
         opts["beta"] = a.beta
+    if a.direction >= 0:
+        opts["direction"] = a.direction
+    if a.mbfs:
+        print(json.dumps(mbfs_bench(a, prep, opts)), flush=True)
+        return
     t0 = time.time()
     dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
     log(f"[bench] device graph: {dg.info()['device_bytes'] / 2**20:.0f} MiB uploaded in {time.time() - t0:.1f}s")

@@ -253,7 +253,16 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
         }
         return;
     }
-    // bottom-up over this workgroup's segment: wave per word, lane per vertex
+    // bottom-up over this workgroup's segment: wave per word, lane per vertex.  A vertex
+    // of degree > kHeavyDeg is deferred to a second pass in which a whole wave scans its
+    // neighbours 64 at a time: with many roots a scan rarely stops early, and one lane
+    // walking a hub's list would hold its wave (and the level) for the whole list.
+    constexpr uint32_t kHeavyDeg = 64;
+    constexpr unsigned kHeavyCap = 2048;
+    __shared__ uint32_t heavyQ[kHeavyCap];
+    __shared__ unsigned heavyN;
+    if (threadIdx.x == 0) heavyN = 0;
+    __syncthreads();
     const Bits<NW> rootMask = root_mask<NW>(a.k);
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
     const unsigned wib = threadIdx.x / kWave, wpb = blockDim.x / kWave;
@@ -270,10 +279,17 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
             U.w[i] = ~vis.w[i] & rootMask.w[i];
             need |= U.w[i] != 0;
         }
+        const uint32_t p0 = need ? g.csrPtr[v] : 0u, p1 = need ? g.csrPtr[v + 1] : 0u;
+        if (need && p1 - p0 > kHeavyDeg) {
+            const unsigned slot = atomicAdd(&heavyN, 1u);
+            if (slot < kHeavyCap) {
+                heavyQ[slot] = v;
+                need = false;  // its frontier word is written by the heavy pass
+            }
+        }
         if (need) {
             Bits<NW> acc;
             acc.zero();
-            const uint32_t p0 = g.csrPtr[v], p1 = g.csrPtr[v + 1];
             for (uint32_t p = p0; p < p1; ++p) {
                 const uint32_t u = g.csrIdx[p];
                 bool all = true;
@@ -299,6 +315,49 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
             }
         }
         nw.store(fout, v);
+    }
+    __syncthreads();
+    const unsigned nh = min(heavyN, kHeavyCap);
+    for (unsigned h = wib; h < nh; h += wpb) {
+        const uint32_t v = heavyQ[h];
+        Bits<NW> vis, U, acc;
+        vis.load(a.visited, v);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) U.w[i] = ~vis.w[i] & rootMask.w[i];
+        acc.zero();
+        const uint32_t p0 = g.csrPtr[v], p1 = g.csrPtr[v + 1];
+        for (uint32_t pb = p0; pb < p1; pb += kWave) {
+            const uint32_t p = pb + lane;
+            if (p < p1) {
+                const uint32_t u = g.csrIdx[p];
+#pragma unroll
+                for (int i = 0; i < NW; ++i) acc.w[i] |= fin[size_t(u) * NW + i];
+            }
+            bool all = true;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                for (int d = kWave / 2; d > 0; d >>= 1) acc.w[i] |= __shfl_xor(acc.w[i], d, kWave);
+                all &= (acc.w[i] & U.w[i]) == U.w[i];
+            }
+            if (all) break;  // acc is the same on every lane: uniform exit
+        }
+        if (lane == 0) {
+            Bits<NW> nw;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) nw.w[i] = acc.w[i] & U.w[i];
+            if (nw.any()) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
+                vis.store(a.visited, v);
+                if (writeLevels)
+                    for (int i = 0; i < NW; ++i)
+                        for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                            a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                news += nw.popc();
+                edges += (unsigned long long)(p1 - p0) * nw.popc();
+            }
+            nw.store(fout, v);
+        }
     }
     news = block_sum_u64(news, tmp64);
     edges = block_sum_u64(edges, tmp64);
@@ -542,7 +601,11 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.ctrl = q.ctrl;
     a.sv = g.seg_view();
     a.sc = q.seg_counters();
-    a.direction = g.hasCsr ? g.opts.direction : 1;
+    // Direction: top-down unless bottom-up is asked for explicitly (direction 2).  With
+    // 64-128 roots a bottom-up scan must find every root still missing at a vertex, so
+    // it seldom stops early; on R-MAT scale 22 (128 roots) the switching schedule took
+    // 21 ms per batch and top-down only 9.6 ms.
+    a.direction = (g.hasCsr && g.opts.direction == 2) ? 2 : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
     a.levelFromCtrl = 0;
