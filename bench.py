@@ -42,6 +42,10 @@ def ensure_built():
         subprocess.run([sys.executable, "-m", "blest_amd.build"], cwd=ROOT, check=True)
 
 
+def info_sym(dg) -> bool:
+    return bool(dg.info()["symmetric"])
+
+
 def harmonic(values):
     v = np.asarray(values, dtype=np.float64)
     return float(len(v) / np.sum(1.0 / v))
@@ -114,9 +118,15 @@ def serve_bench(a) -> dict:
         seq_service[i] = time.perf_counter() - t1
     del qs
     log(f"[serve] one-at-a-time: {N} requests in {seq_service.sum():.2f}s")
+    for gi, (label, _, _, _) in enumerate(graphs):
+        for k, kname in ((0, "bfs"), (1, "cc")):
+            sel = (gsel == gi) & (kinds == k)
+            if sel.any():
+                log(f"[serve]   {label} {kname}: {int(sel.sum())} requests, mean {seq_service[sel].mean() * 1e3:.3f} ms, "
+                    f"max {seq_service[sel].max() * 1e3:.3f} ms")
 
     srv = B.Server([dg for _, dg, _, _ in graphs], workers=a.serve_workers, max_batch=a.serve_batch,
-                   capacity=per_graph_bufs * len(graphs))
+                   capacity=per_graph_bufs * len(graphs), min_multi=a.serve_min_multi)
     free = [queue.SimpleQueue() for _ in graphs]
     for gi, bl in enumerate(bufs):
         for j in range(len(bl)):
@@ -192,7 +202,8 @@ def serve_bench(a) -> dict:
             "data": "synthetic",
             "config": {"workload": "serve", "graphs": [x[0] for x in graphs], "reorder": "natural",
                        "mix": "80% single-root BFS, 20% CC", "requests_per_rate": N,
-                       "workers": a.serve_workers, "max_batch": a.serve_batch, "results": "device buffers"},
+                       "workers": a.serve_workers, "max_batch": a.serve_batch, "min_multi": a.serve_min_multi,
+                       "results": "device buffers"},
             "rates": rows, "roofline": None, "cpu_baseline": None}
 
 
@@ -328,6 +339,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-cc", action="store_true", help="skip the connected-components timing")
     ap.add_argument("--roofline-roots", type=int, default=-1,
                     help="roots replayed for the byte model (-1: all up to 2^23 vertices, else 8; 0: all)")
     ap.add_argument("--block-size", type=int, default=0)
@@ -341,6 +353,7 @@ def main():
     ap.add_argument("--serve-rates", type=lambda s: [int(x) for x in s.split(",")], default=None)
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--serve-min-multi", type=int, default=8)
     ap.add_argument("--serve-buffers", type=int, default=48, help="serve: device result buffers per graph")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--mbfs", action="store_true", help="batched multi-source BFS vs plain on the workload's roots")
@@ -497,6 +510,20 @@ def main():
                          f"direction rule, {threads} OpenMP threads"}
         log(f"[bench] cpu baseline {cpu['value']} GTEPS over {len(cr)} roots")
 
+    cc = None
+    if not a.no_cc and info_sym(dg):
+        # connected components on the same resident graph (canonical labels stay on the device)
+        import ctypes as C
+        cs = B._lib.CcStats()
+        B.check(lib.blest_cc(q, 0, None, C.byref(cs)))  # warm-up (allocates the label arrays)
+        cc_ms = []
+        for _ in range(max(1, a.steps)):
+            B.check(lib.blest_cc(q, 0, None, C.byref(cs)))
+            cc_ms.append(cs.kernel_ms)
+        m = dg.info()["nnz"] / 2
+        cc = {"ms": round(float(np.mean(cc_ms)), 4), "components": cs.no_components,
+              "medges_per_s": round(m / (np.mean(cc_ms) * 1e-3) / 1e6, 1), "algo": "hook+compress"}
+        log(f"[bench] cc: {cc}")
     info = dg.info()
     cfg = {"workload": w.name}
     if w.kind == "rmat":
@@ -510,7 +537,7 @@ def main():
     line = {"metric": f"BFS GTEPS (harmonic mean, {len(roots)} roots)", "value": round(value, 3), "unit": "GTEPS",
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-            "config": cfg, "roofline": roofline, "cpu_baseline": cpu}
+            "config": cfg, "roofline": roofline, "cpu_baseline": cpu, "cc": cc}
     print(json.dumps(line), flush=True)
 
 

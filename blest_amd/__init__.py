@@ -484,11 +484,12 @@ class Server:
     KINDS = {"bfs": 0, "cc": 1}
 
     def __init__(self, graphs: list[DeviceGraph], workers: int = 4, max_batch: int = 64,
-                 hbm_budget: int = 0, capacity: int = 65536):
+                 hbm_budget: int = 0, capacity: int = 65536, min_multi: int = 8):
         self._lib = _lib.load()
         o = _lib.ServerOpts()
         self._lib.blest_server_opts_default(C.byref(o))
         o.workers, o.max_batch, o.hbm_budget, o.capacity = workers, max_batch, hbm_budget, capacity
+        o.min_multi = min_multi
         arr = (C.c_void_p * len(graphs))(*[g._h.value for g in graphs])
         h = C.c_void_p()
         check(self._lib.blest_server_create(arr, len(graphs), C.byref(o), C.byref(h)))

@@ -72,7 +72,7 @@ class CcStats(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("workers", C.c_uint32), ("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64),
-                ("capacity", C.c_uint32)]
+                ("capacity", C.c_uint32), ("min_multi", C.c_uint32)]
 
 
 class RequestResult(C.Structure):

@@ -177,10 +177,11 @@ struct Server {
         std::vector<blest_bfs_stats> perRoot(k);
         blest_cc_stats cs{};
         PooledQuery* pq = nullptr;
+        const bool multi = !cc && k > 1 && k >= opts.min_multi;
         try {
             std::unique_lock<std::mutex> lk(mu);
-            // a batch of one uses the single-source state; larger ones the batch capacity
-            pq = acquire(lk, g, k == 1 ? 1 : opts.max_batch);
+            // single-source runs use single-source state; a multi-source launch the batch capacity
+            pq = acquire(lk, g, multi ? opts.max_batch : 1);
         } catch (const Error& e) {
             st = e.code;
             why = e.what();
@@ -188,9 +189,11 @@ struct Server {
         if (pq) {
             if (cc) {
                 st = blest_cc(pq->q, 0, static_cast<uint32_t*>(batch[0]->out), &cs);
-            } else if (k == 1) {
-                st = blest_bfs(pq->q, batch[0]->root, static_cast<int32_t*>(batch[0]->out), &bs);
-                perRoot[0] = bs;
+            } else if (!multi) {
+                for (uint32_t j = 0; j < k && st == BLEST_OK; ++j) {
+                    st = blest_bfs(pq->q, batch[j]->root, static_cast<int32_t*>(batch[j]->out), &perRoot[j]);
+                    bs = perRoot[j];
+                }
             } else {
                 std::vector<uint32_t> roots(k);
                 std::vector<int32_t*> rows(k);
@@ -258,6 +261,7 @@ extern "C" void blest_server_opts_default(blest_server_opts* o) {
     o->max_batch = 64;
     o->hbm_budget = 0;
     o->capacity = 65536;
+    o->min_multi = 8;
 }
 
 extern "C" blest_status blest_server_create(blest_device_graph* const* graphs, uint32_t count,

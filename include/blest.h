@@ -299,6 +299,9 @@ typedef struct blest_server_opts {
     uint32_t max_batch;      /* roots per multi-source launch (<= 128) */
     uint64_t hbm_budget;     /* bytes for query state; 0 = 80% of free HBM at creation */
     uint32_t capacity;       /* max queued requests (BLEST_ERR_BUSY beyond) */
+    uint32_t min_multi;      /* BFS batches smaller than this run as single-source
+                                traversals one after another (a multi-source launch
+                                costs about as much as 8 single-source ones) */
 } blest_server_opts;
 void blest_server_opts_default(blest_server_opts* o);
 typedef enum blest_request_kind { BLEST_REQ_BFS = 0, BLEST_REQ_CC = 1 } blest_request_kind;

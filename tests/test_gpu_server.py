@@ -44,11 +44,11 @@ def check(oracle, graphs, reqs, results):
     return batched
 
 
-@pytest.mark.parametrize("workers,max_batch", [(1, 1), (4, 64), (2, 128)])
-def test_server_mixed_stream(blest, oracle, workers, max_batch):
-    say(f"case server workers={workers} max_batch={max_batch}")
+@pytest.mark.parametrize("workers,max_batch,min_multi", [(1, 1, 8), (4, 64, 8), (2, 128, 2), (3, 64, 1)])
+def test_server_mixed_stream(blest, oracle, workers, max_batch, min_multi):
+    say(f"case server workers={workers} max_batch={max_batch} min_multi={min_multi}")
     graphs = make_graphs(blest)
-    srv = blest.Server([g[3] for g in graphs], workers=workers, max_batch=max_batch)
+    srv = blest.Server([g[3] for g in graphs], workers=workers, max_batch=max_batch, min_multi=min_multi)
     rng = np.random.default_rng(workers * 131 + max_batch)
     reqs = []
     for i in range(300):
@@ -76,7 +76,7 @@ def test_server_eviction_budget(blest, oracle):
         b = C.c_uint64()
         blest._lib.check(lib.blest_query_footprint(g[3]._h, 16, C.byref(b)))
         need.append(b.value)
-    srv = blest.Server([g[3] for g in graphs], workers=3, max_batch=16, hbm_budget=max(need))
+    srv = blest.Server([g[3] for g in graphs], workers=3, max_batch=16, hbm_budget=max(need), min_multi=2)
     rng = np.random.default_rng(9)
     reqs = [(i % 3, "bfs", int(rng.integers(len(graphs[i % 3][1]) - 1))) for i in range(120)]
     tickets = [srv.submit(*r) for r in reqs]
