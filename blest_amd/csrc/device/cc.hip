@@ -101,35 +101,19 @@ __global__ void __launch_bounds__(256) cc_hook_kernel(GraphView g, uint32_t* p) 
     }
 }
 
-// After hooking the roots are final.  Each thread moves only its own pointer, to its
-// current grandparent, until its parent is a root: no thread ever stores into another
-// vertex's entry (a path-halving store there could overwrite that vertex's final root
-// with a non-root ancestor), every value read is an ancestor, and since all pointers
-// jump concurrently the walk takes about log(depth) steps instead of depth.
+// After hooking the roots are final.  Compression must store roots only: a
+// path-halving store (an ancestor that is not the root) racing with another thread's
+// final store could overwrite it.  Every value a thread can read here (an old parent or
+// a root) is an ancestor, so the read-only walk always ends at the root.
 __global__ void cc_compress_kernel(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t par = ld_parent(p, i);
-        for (;;) {
-            const uint32_t gp = ld_parent(p, par);
-            if (gp == par) break;
-            __hip_atomic_store(p + i, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            par = gp;
+        uint32_t x = i, par = ld_parent(p, x);
+        while (par != x) {
+            x = par;
+            par = ld_parent(p, x);
         }
+        if (x != i) __hip_atomic_store(p + i, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-
-// Components = vertices that are their own label (labels[v] == v), counted on the
-// device: the caller's label array may be device memory.
-__global__ void cc_count_clear_kernel(Ctrl* c) {
-    if (threadIdx.x == 0) c->visited = 0;
-}
-
-__global__ void __launch_bounds__(256) cc_count_kernel(const uint32_t* labels, uint32_t n, Ctrl* c) {
-    unsigned cnt = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        cnt += labels[i] == i;
-    cnt = wave_sum_u32(cnt);
-    if (lane_id() == 0 && cnt) atomicAdd(&c->visited, cnt);
 }
 
 __global__ void fill_u32_kernel(uint32_t* a, uint32_t v, uint32_t n) {
@@ -283,6 +267,15 @@ __global__ void __launch_bounds__(kProducerBlock) cc_lp_update_kernel(GraphView 
 }  // namespace dev
 
 // ------------------------------------------------------------------------ host
+static bool is_device_pointer(const void* p) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();  // unregistered host memory
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
 static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st) {
     DeviceGraph& g = *q.g;
     if (!g.symmetric) throw Error(BLEST_ERR_UNSUPPORTED, "connected components need a symmetric graph");
@@ -340,15 +333,20 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
         BLEST_HIP(hipGetLastError());
         result = reinterpret_cast<const uint32_t*>(q.levels);
     }
-    hipLaunchKernelGGL(dev::cc_count_clear_kernel, dim3(1), dim3(64), 0, q.stream, q.ctrl);
-    hipLaunchKernelGGL(dev::cc_count_kernel, grid, block, 0, q.stream, result, g.n, q.ctrl);
-    BLEST_HIP(hipGetLastError());
-    unsigned comps = 0;
-    BLEST_HIP(hipMemcpyAsync(&comps, &q.ctrl->visited, sizeof(unsigned), hipMemcpyDeviceToHost, q.stream));
-    if (hostLabels)  // host or device memory
-        BLEST_HIP(hipMemcpyAsync(hostLabels, result, 4 * size_t(g.n), hipMemcpyDefault, q.stream));
+    // Components = vertices that are their own label, counted on the host from host
+    // memory only: when the caller's array is device memory the count reads a host copy.
+    if (hostLabels) BLEST_HIP(hipMemcpyAsync(hostLabels, result, 4 * size_t(g.n), hipMemcpyDefault, q.stream));
+    std::vector<uint32_t> tmp;
+    const uint32_t* counted = hostLabels;
+    if (!hostLabels || is_device_pointer(hostLabels)) {
+        tmp.resize(g.n);
+        BLEST_HIP(hipMemcpyAsync(tmp.data(), result, 4 * size_t(g.n), hipMemcpyDeviceToHost, q.stream));
+        counted = tmp.data();
+    }
     BLEST_HIP(hipStreamSynchronize(q.stream));
     if (st) {
+        uint32_t comps = 0;
+        for (uint32_t v = 0; v < g.n; ++v) comps += counted[v] == v;
         st->kernel_ms = ms;
         st->no_components = comps;
         st->iterations = iterations;
