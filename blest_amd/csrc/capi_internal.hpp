@@ -24,6 +24,10 @@ const BVSS& bvss_of(const blest_bvss* h);
 blest_status mbfs_rows(blest_query* q, const uint32_t* srcs, uint32_t k, int32_t* const* rows,
                        blest_bfs_stats* st, blest_bfs_stats* perRoot);
 
+// Marks a query as never replaying captured level chunks (used by the request server,
+// whose worker threads allocate while others run).
+void query_disable_graphs(blest_query* q);
+
 // Runs fn, converting exceptions into a status code (and the thread's last error).
 template <class F>
 blest_status guarded(F&& fn) {

@@ -89,7 +89,7 @@ void run_levels(Query& q, const std::function<void(unsigned)>& level, const std:
                 LevelGraph* lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
     const unsigned limit = q.g->n + 2;
     const int mode = q.g->opts.use_graphs;
-    if (lg && levelRel && (mode == 1 || (mode == 0 && q.levelHint >= kGraphMinLevels))) {
+    if (lg && levelRel && !q.noGraphs && (mode == 1 || (mode == 0 && q.levelHint >= kGraphMinLevels))) {
         if (!lg->exec || lg->key != key) capture_levels(q, *lg, levelRel, key);
         const unsigned C = lg->chunk;
         unsigned launched = 0;
@@ -133,7 +133,7 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->direction = 0;
     o->keep_csr = 1;
     o->timeout_ms = 20000;
-    o->use_graphs = 0;
+    o->use_graphs = 2;
 }
 
 blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,
@@ -386,3 +386,7 @@ blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_so
 
 
 }  // extern "C"
+
+void blest::query_disable_graphs(blest_query* q) {
+    if (q) q->q.noGraphs = true;
+}

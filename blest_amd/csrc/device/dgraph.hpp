@@ -131,6 +131,7 @@ struct Query {
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries
     LevelGraph graphs[kGraphCount];
+    bool noGraphs = false;  // never replay captured level chunks (queries of the request server)
     std::vector<hipEvent_t> traceEvents;  // one per traced kernel boundary (created on demand)
     std::vector<void*> allocations;
     std::mutex mu;
@@ -158,7 +159,7 @@ struct Query {
 // Level-loop driver: enqueues level(L) for L = 1, 2, ... in chunks and polls the
 // device's done flag after each chunk.  With lg and levelRel given (levelRel(k) launches
 // level levelBase + k), deep traversals replay a captured chunk of levels instead
-// (device opts use_graphs: 0 auto, 1 always, 2 never).
+// (device opts use_graphs: 0 auto, 1 always, 2 never; queries with noGraphs never).
 dim3 level_grid(const DeviceGraph& g);
 void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail = nullptr,
                 LevelGraph* lg = nullptr, const std::function<void(unsigned)>& levelRel = nullptr,

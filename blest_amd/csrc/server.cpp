@@ -156,6 +156,7 @@ struct Server {
                 cvPool.notify_all();
                 throw Error(st, why);
             }
+            query_disable_graphs(q);
             pool.push_back(PooledQuery{q, graph, capacity, need, true, ++useClock});
             return &pool.back();
         }
