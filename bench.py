@@ -347,6 +347,7 @@ def main():
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
+    ap.add_argument("--autotune", action="store_true", help="pick blocks_per_cu / alpha / beta with the autotuner")
     ap.add_argument("--serve-scales", default="", help="serve: comma-separated R-MAT scales (default 18..25)")
     ap.add_argument("--serve-grids", default="", help="serve: comma-separated grid sides (default 2048,4096)")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -398,6 +399,16 @@ def main():
     if a.mbfs:
         print(json.dumps(mbfs_bench(a, prep, opts)), flush=True)
         return
+    tuned = None
+    if a.autotune:
+        # launch shape and switch thresholds chosen by the native autotuner on 8 of the roots
+        t0 = time.time()
+        best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:8], **opts)
+        for k in ("blocks_per_cu", "alpha", "beta"):
+            opts[k] = best[k]
+        tuned = {"blocks_per_cu": best["blocks_per_cu"], "alpha": best["alpha"], "beta": best["beta"],
+                 "trials": trials, "tuning_ms_per_bfs": round(best_ms, 4)}
+        log(f"[bench] autotuned in {time.time() - t0:.1f}s: {tuned}")
     t0 = time.time()
     dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
     log(f"[bench] device graph: {dg.info()['device_bytes'] / 2**20:.0f} MiB uploaded in {time.time() - t0:.1f}s")
@@ -532,6 +543,8 @@ def main():
     else:
         cfg.update({"graph": f"{w.side}x{w.side} grid keep {w.keep} shortcuts {w.shortcut_frac} seed {w.seed}",
                     "vertices": info["n"], "directed_edges": info["nnz"]})
+    if tuned:
+        cfg["autotuned"] = tuned
     cfg.update({"reorder": f"{w.reorder.name.lower()} window {w.window}", "roots": len(roots),
                 "slices": info["slices"], "vsets": info["vsets"]})
     line = {"metric": f"BFS GTEPS (harmonic mean, {len(roots)} roots)", "value": round(value, 3), "unit": "GTEPS",

@@ -74,8 +74,8 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
             }
         }
         if (bestOpts.direction == 0 && keep.dg) {
-            static const double alphas[] = {4, 8, 15, 30, 60, 120};
-            static const double betas[] = {6, 12, 18, 24, 48, 96};
+            static const double alphas[] = {4, 8, 15, 30, 60, 120, 250, 500, 1000};
+            static const double betas[] = {6, 12, 18, 24, 48, 96, 200, 400, 800};
             for (double a : alphas) {
                 if (a == bestOpts.alpha) continue;
                 check(blest_device_graph_set_thresholds(keep.dg, a, bestOpts.beta, 0));

@@ -199,82 +199,83 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
     }
 }
 
+// One pass, thread per rset: the block's segment is walked in chunks of blockDim/8
+// words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
+// fold `next` into visited / frontier (the word's first thread), write the new
+// vertices' levels and sum their degrees; both directions then emit the active rsets'
+// vsets (one block scan per chunk, entries written by all threads with an owner
+// cursor).  All of a thread's loads in a chunk are independent, so a level costs a few
+// memory round trips whatever the frontier size.
 __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[16];
+    __shared__ unsigned long long tmp64[16];
     __shared__ unsigned sEx[kProducerBlock + 1];
     __shared__ uint32_t sStart[kProducerBlock];
-    __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
     const bool bottomUp = c->state[L & 1].dir != 0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
-    uint32_t* lout = (L & 1) ? a.list0 : a.list1;
+    uint32_t* out = ((L & 1) ? a.list0 : a.list1) + a.sv.segBase[blockIdx.x];
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
-    const unsigned lane = lane_id(), wib = threadIdx.x / kWave, wpb = blockDim.x / kWave;
-    uint32_t first, last;
-    wave_words(ws, we, wib, wpb, first, last);
+    const unsigned t = threadIdx.x, b = t & 7, nb = blockDim.x;
     const bool canBU = g.csrPtr != nullptr;
     unsigned long long news = 0, edges = 0;
-    // pass 1 (top-down levels): lane per word (one coalesced load per 64 words) folds
-    // `next` into visited / frontier; the new vertices' levels are then written lane per
-    // vertex (coalesced), four words' degree loads in flight at once.
-    for (uint32_t c0 = first; c0 < last && !bottomUp; c0 += kWave) {
-        const uint32_t cw = c0 + lane;
-        unsigned long long mine = 0;
-        if (cw < last) {
-            mine = a.next[cw];
-            if (mine) {
-                a.next[cw] = 0ull;
-                a.visited[cw] |= mine;
-            }
-            fout[cw] = mine;
-        }
-        news += __popcll(mine);
-        unsigned long long nz = __ballot(mine != 0);
-        while (nz) {
-            unsigned js[4];
-            unsigned long long nws[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                js[t] = 0;
-                nws[t] = 0;
-                if (nz) {
-                    js[t] = __builtin_ctzll(nz);
-                    nz &= nz - 1;
-                    nws[t] = __shfl(mine, js[t], kWave);
+    unsigned written = 0;
+    for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
+        const uint32_t w = c0 + t / 8, r = w * 8 + b;
+        uint32_t start = 0, cnt = 0;
+        if (w < we) {
+            unsigned long long word;
+            if (bottomUp) {
+                word = fout[w];
+            } else {
+                // the word's 8 threads are lanes of one wave: all load before its store
+                word = a.next[w];
+                if (b == 0) {
+                    if (word) {
+                        a.next[w] = 0ull;
+                        a.visited[w] |= word;
+                    }
+                    fout[w] = word;
                 }
             }
-            uint32_t lo[4], hi[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                lo[t] = hi[t] = 0;
-                if ((nws[t] >> lane) & 1ull) {
-                    const uint32_t v = (c0 + js[t]) * 64 + lane;
-                    a.levels[v] = int32_t(L);
-                    if (canBU) {
-                        lo[t] = g.csrPtr[v];
-                        hi[t] = g.csrPtr[v + 1];
+            const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
+            if (byte) {
+                start = g.rsetUnit[r];
+                cnt = g.rsetUnit[r + 1] - start;
+                if (!bottomUp) {
+                    news += __popc(byte);
+                    for (uint32_t bits = byte; bits; bits &= bits - 1) {
+                        const uint32_t v = r * 8 + __builtin_ctz(bits);
+                        a.levels[v] = int32_t(L);
+                        if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
                     }
                 }
             }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) edges += hi[t] - lo[t];
         }
+        unsigned total;
+        const unsigned ex = block_exclusive_scan(cnt, tmp, total);
+        sEx[t] = ex;
+        sStart[t] = start;
+        if (t == 0) sEx[nb] = total;
+        __syncthreads();
+        unsigned cur = 0;
+        for (unsigned e = t; e < total; e += nb) {
+            while (sEx[cur + 1] <= e) ++cur;
+            out[written + e] = sStart[cur] + (e - sEx[cur]);
+        }
+        written += total;
+        __syncthreads();  // the next chunk reuses sEx / sStart
     }
-    __syncthreads();  // pass 1's frontier words are read by other waves below
-    // pass 2: the segment's vset list, written by the whole workgroup
-    const unsigned running = block_write_vsets(
-        g.rsetUnit, ws, we, [](uint32_t) {}, [&](uint32_t w) { return fout[w]; },
-        lout + a.sv.segBase[blockIdx.x], sEx, sStart, tmp);
     if (!bottomUp) {
         news = block_sum_u64(news, tmp64);
         edges = block_sum_u64(edges, tmp64);
     }
-    if (threadIdx.x == 0) {
-        a.sc.count[size_t(sOut) * segs + blockIdx.x] = running;
+    if (t == 0) {
+        a.sc.count[size_t(sOut) * segs + blockIdx.x] = written;
         if (!bottomUp) {
             a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(news);
             a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;

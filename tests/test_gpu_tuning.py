@@ -18,7 +18,8 @@ def test_autotune_then_parity(blest, oracle):
     best, ms, trials = blest.autotune_bfs(b, g, inv, roots_orig)
     assert trials >= 4 and ms > 0
     assert best["blocks_per_cu"] in (1, 2, 4, 8)
-    assert best["alpha"] in (4, 8, 15, 30, 60, 120) and best["beta"] in (6, 12, 18, 24, 48, 96)
+    assert best["alpha"] in (4, 8, 15, 30, 60, 120, 250, 500, 1000)
+    assert best["beta"] in (6, 12, 18, 24, 48, 96, 200, 400, 800)
     dg = blest.DeviceGraph(b, g, inv, **{k: best[k] for k in ("blocks_per_cu", "alpha", "beta")})
     k = blest.BVSSBFSKernel(dg)
     for r in roots_orig:
