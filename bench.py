@@ -401,9 +401,9 @@ def main():
         return
     tuned = None
     if a.autotune:
-        # launch shape and switch thresholds chosen by the native autotuner on 8 of the roots
+        # launch shape and switch thresholds chosen by the native autotuner on 32 of the roots
         t0 = time.time()
-        best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:8], **opts)
+        best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:32], **opts)
         for k in ("blocks_per_cu", "alpha", "beta"):
             opts[k] = best[k]
         tuned = {"blocks_per_cu": best["blocks_per_cu"], "alpha": best["alpha"], "beta": best["beta"],

@@ -119,10 +119,14 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
         const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
         const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
-        for (unsigned base = wave * kWave; base < total; base += nwaves * kWave) {
+        // m list entries per wave step: a short list of large vsets (the second level
+        // of an R-MAT traversal: a few thousand vsets of ~100 slices) is spread over as
+        // many waves as it has entries instead of 64 entries per wave
+        const unsigned m = min(unsigned(kWave), max(1u, (total + nwaves - 1) / nwaves));
+        for (unsigned base = wave * m; base < total; base += nwaves * m) {
             const unsigned e = base + lane;
             uint32_t q0 = 0, qc = 0, fb = 0;
-            if (e < total) {
+            if (lane < m && e < total) {
                 const uint32_t v = list_entry(lin, a.sv, pre, e);
                 q0 = g.unitQuad[v];
                 qc = g.unitQuad[v + 1] - q0;
@@ -203,8 +207,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
 // words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
-// vsets (one block scan per chunk, entries written by all threads with an owner
-// cursor).  All of a thread's loads in a chunk are independent, so a level costs a few
+// vsets (one block scan per chunk, entries written by all threads, each entry's owner
+// found by binary search).  All of a thread's loads in a chunk are independent, so a level costs a few
 // memory round trips whatever the frontier size.
 __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
@@ -262,10 +266,9 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
-        unsigned cur = 0;
         for (unsigned e = t; e < total; e += nb) {
-            while (sEx[cur + 1] <= e) ++cur;
-            out[written + e] = sStart[cur] + (e - sEx[cur]);
+            const unsigned o = block_owner(sEx, nb, e);
+            out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
         __syncthreads();  // the next chunk reuses sEx / sStart

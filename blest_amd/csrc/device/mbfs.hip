@@ -198,10 +198,12 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
         const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
         const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
-        for (unsigned base = wave * kWave; base < total; base += nwaves * kWave) {
+        // m list entries per wave step (a short list is spread over as many waves)
+        const unsigned m = min(unsigned(kWave), max(1u, (total + nwaves - 1) / nwaves));
+        for (unsigned base = wave * m; base < total; base += nwaves * m) {
             const unsigned e = base + lane;
             uint32_t q0 = 0, qc = 0, r = 0;
-            if (e < total) {
+            if (lane < m && e < total) {
                 const uint32_t vs = list_entry(lin, a.sv, pre, e);
                 q0 = g.unitQuad[vs];
                 qc = g.unitQuad[vs + 1] - q0;

@@ -163,10 +163,22 @@ __device__ __forceinline__ unsigned wave_write_vsets(const uint32_t* rsetUnit, u
     return written;
 }
 
+// Largest r in [0, nb) with ex[r] <= e, given ex ascending, ex[0] == 0 and
+// ex[nb] > e: the rset that owns entry e (rsets with no entries are skipped).
+__device__ __forceinline__ unsigned block_owner(const unsigned* ex, unsigned nb, unsigned e) {
+    unsigned lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (ex[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Block-wide form for a producer workgroup's whole segment [ws, we): thread = one rset
 // of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
-// are written by all threads, entry e by thread e % blockDim (coalesced), each thread
-// walking an owner cursor over the chunk's prefix in LDS.  A segment holding hub rsets
+// are written by all threads, entry e by thread e % blockDim (coalesced), its owner
+// found by binary search over the chunk's prefix in LDS.  A segment holding hub rsets
 // (hundreds of thousands of vsets) is so spread over the whole workgroup.  prep(c0) runs
 // on every thread before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds
 // blockDim + 1 entries, sStart blockDim.  Returns the segment's entry count.
@@ -193,10 +205,9 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
-        unsigned cur = 0;
         for (unsigned e = t; e < total; e += nb) {
-            while (sEx[cur + 1] <= e) ++cur;
-            out[written + e] = sStart[cur] + (e - sEx[cur]);
+            const unsigned o = block_owner(sEx, nb, e);
+            out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
         __syncthreads();  // the next chunk reuses sEx / sStart
