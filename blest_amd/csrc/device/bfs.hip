@@ -207,8 +207,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
 // words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
-// vsets (one block scan per chunk, entries written by all threads, each entry's owner
-// found by binary search).  All of a thread's loads in a chunk are independent, so a level costs a few
+// vsets (one block scan per chunk, entries written by all threads: binary search for a
+// thread's first owner, then a forward cursor).  All of a thread's loads in a chunk are independent, so a level costs a few
 // memory round trips whatever the frontier size.
 __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
@@ -252,11 +252,15 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
                 cnt = g.rsetUnit[r + 1] - start;
                 if (!bottomUp) {
                     news += __popc(byte);
-                    for (uint32_t bits = byte; bits; bits &= bits - 1) {
-                        const uint32_t v = r * 8 + __builtin_ctz(bits);
-                        a.levels[v] = int32_t(L);
-                        if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
+                    // degree loads first (all independent, unrolled), then the level stores
+                    if (canBU) {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if ((byte >> k) & 1u) edges += g.csrPtr[r * 8 + k + 1] - g.csrPtr[r * 8 + k];
                     }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if ((byte >> k) & 1u) a.levels[r * 8 + k] = int32_t(L);
                 }
             }
         }
@@ -266,8 +270,11 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
+        // owner of the first entry by binary search, then a forward cursor: a hub rset
+        // keeps the cursor in place for hundreds of entries
+        unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
         for (unsigned e = t; e < total; e += nb) {
-            const unsigned o = block_owner(sEx, nb, e);
+            while (sEx[o + 1] <= e) ++o;
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;

@@ -177,8 +177,9 @@ __device__ __forceinline__ unsigned block_owner(const unsigned* ex, unsigned nb,
 
 // Block-wide form for a producer workgroup's whole segment [ws, we): thread = one rset
 // of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
-// are written by all threads, entry e by thread e % blockDim (coalesced), its owner
-// found by binary search over the chunk's prefix in LDS.  A segment holding hub rsets
+// are written by all threads, entry e by thread e % blockDim (coalesced); a thread finds
+// the owner of its first entry by binary search over the chunk's prefix in LDS and then
+// moves a cursor forward.  A segment holding hub rsets
 // (hundreds of thousands of vsets) is so spread over the whole workgroup.  prep(c0) runs
 // on every thread before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds
 // blockDim + 1 entries, sStart blockDim.  Returns the segment's entry count.
@@ -205,8 +206,11 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
+        // owner of the first entry by binary search, then a forward cursor: a hub rset
+        // keeps the cursor in place for hundreds of entries
+        unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
         for (unsigned e = t; e < total; e += nb) {
-            const unsigned o = block_owner(sEx, nb, e);
+            while (sEx[o + 1] <= e) ++o;
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
