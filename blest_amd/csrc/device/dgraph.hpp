@@ -127,6 +127,7 @@ struct Query {
     uint32_t mbfsWords = 0;
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
     unsigned long long* mHist = nullptr;  // frontier words of levels 0..mHistLevels
+    unsigned long long* mFull = nullptr;  // rows every root has reached (1 bit per vertex)
     uint32_t mHistLevels = 0;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries

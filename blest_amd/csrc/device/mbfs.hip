@@ -24,12 +24,12 @@ namespace dev {
 
 struct MbfsArgs {
     int32_t* levels;
-    unsigned long long* visited;
+    unsigned long long* vn;  // per vertex: NW visited words, then NW `next` words (one line)
     unsigned long long* front0;  // frontier words of levels past the history, by parity
     unsigned long long* front1;
     unsigned long long* hist;    // frontier words of levels 0..histLevels (n * NW each)
     uint32_t histLevels;
-    unsigned long long* next;
+    unsigned long long* full;    // bit v: every root has reached v (top-down skips its record)
     uint32_t* list0;
     uint32_t* list1;
     const uint32_t* srcs;
@@ -69,7 +69,26 @@ struct Bits {
 #pragma unroll
         for (int i = 0; i < NW; ++i) p[size_t(v) * NW + i] = w[i];
     }
+    __device__ __forceinline__ void loadp(const unsigned long long* p) {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) w[i] = p[i];
+    }
+    __device__ __forceinline__ void storep(unsigned long long* p) const {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) p[i] = w[i];
+    }
 };
+
+// A vertex's visited and `next` words sit side by side (2 x NW words, 32 B at 128 roots),
+// so the random row access of a top-down slice touches one cache line, not two.
+template <int NW>
+__device__ __forceinline__ unsigned long long* vis_at(const MbfsArgs& a, uint32_t v) {
+    return a.vn + size_t(v) * (2 * NW);
+}
+template <int NW>
+__device__ __forceinline__ unsigned long long* next_at(const MbfsArgs& a, uint32_t v) {
+    return a.vn + size_t(v) * (2 * NW) + NW;
+}
 
 template <int NW>
 __device__ __forceinline__ Bits<NW> root_mask(uint32_t k) {
@@ -134,10 +153,11 @@ template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_init_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     unsigned long long* f0 = a.hist;  // level 0
+    for (uint32_t w = tid; w < g.W; w += nthreads) a.full[w] = 0ull;
     for (size_t i = tid; i < size_t(g.n) * NW; i += nthreads) {
-        a.visited[i] = 0ull;
         f0[i] = 0ull;
-        a.next[i] = 0ull;
+        a.vn[2 * i] = 0ull;
+        a.vn[2 * i + 1] = 0ull;
     }
     for (uint32_t s = tid; s < a.sv.segs; s += nthreads) {
         a.sc.news[a.sv.segs + s] = 0;
@@ -156,7 +176,7 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
     if (j >= a.k) return;
     const uint32_t v = a.srcs[j];
     const unsigned long long bit = 1ull << (j & 63);
-    atomicOr(&a.visited[size_t(v) * NW + j / 64], bit);
+    atomicOr(vis_at<NW>(a, v) + j / 64, bit);
     atomicOr(&a.hist[size_t(v) * NW + j / 64], bit);
     a.levels[size_t(j) * g.n + v] = 0;
     const uint32_t seg = (v >> 6) / a.sv.wps;
@@ -243,11 +263,13 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
                                 for (int i = 0; i < NW; ++i) acc.w[i] |= F[b].w[i];
                         if (!acc.any()) continue;
                         const uint32_t row = rowv[s];
+                        if ((a.full[row >> 6] >> (row & 63)) & 1ull) continue;  // L2-resident bitmap
+                        const unsigned long long* vr = vis_at<NW>(a, row);
+                        unsigned long long* nr = next_at<NW>(a, row);
 #pragma unroll
                         for (int i = 0; i < NW; ++i) {
-                            const size_t at = size_t(row) * NW + i;
-                            const unsigned long long nv = acc.w[i] & ~(a.visited[at] | a.next[at]);
-                            if (nv) atomicOr(&a.next[at], nv);
+                            const unsigned long long nv = acc.w[i] & ~(vr[i] | nr[i]);
+                            if (nv) atomicOr(nr + i, nv);
                         }
                     }
                 }
@@ -273,7 +295,7 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
         const uint32_t v = w * 64 + lane;
         if (v >= g.n) continue;
         Bits<NW> vis, U, nw;
-        vis.load(a.visited, v);
+        vis.loadp(vis_at<NW>(a, v));
         nw.zero();
         bool need = false;
 #pragma unroll
@@ -307,7 +329,7 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
             if (nw.any()) {
 #pragma unroll
                 for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
-                vis.store(a.visited, v);
+                vis.storep(vis_at<NW>(a, v));
                 if (writeLevels)
                     for (int i = 0; i < NW; ++i)
                         for (unsigned long long b = nw.w[i]; b; b &= b - 1)
@@ -323,7 +345,7 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
     for (unsigned h = wib; h < nh; h += wpb) {
         const uint32_t v = heavyQ[h];
         Bits<NW> vis, U, acc;
-        vis.load(a.visited, v);
+        vis.loadp(vis_at<NW>(a, v));
 #pragma unroll
         for (int i = 0; i < NW; ++i) U.w[i] = ~vis.w[i] & rootMask.w[i];
         acc.zero();
@@ -350,7 +372,7 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
             if (nw.any()) {
 #pragma unroll
                 for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
-                vis.store(a.visited, v);
+                vis.storep(vis_at<NW>(a, v));
                 if (writeLevels)
                     for (int i = 0; i < NW; ++i)
                         for (unsigned long long b = nw.w[i]; b; b &= b - 1)
@@ -392,20 +414,26 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     uint32_t first, last;
     wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
     unsigned long long news = 0, edges = 0;
+    const Bits<NW> rootMask = root_mask<NW>(a.k);
     for (uint32_t w = first; w < last; ++w) {
         const uint32_t v = w * 64 + lane_id();
         Bits<NW> nw;
         nw.zero();
+        bool nowFull = false;
         if (v < g.n) {
-            nw.load(a.next, v);
+            nw.loadp(next_at<NW>(a, v));
             if (nw.any()) {
                 Bits<NW> vis, z;
-                vis.load(a.visited, v);
+                vis.loadp(vis_at<NW>(a, v));
                 z.zero();
-                z.store(a.next, v);
+                z.storep(next_at<NW>(a, v));
+                nowFull = true;
 #pragma unroll
-                for (int i = 0; i < NW; ++i) vis.w[i] |= nw.w[i];
-                vis.store(a.visited, v);
+                for (int i = 0; i < NW; ++i) {
+                    vis.w[i] |= nw.w[i];
+                    nowFull &= vis.w[i] == rootMask.w[i];
+                }
+                vis.storep(vis_at<NW>(a, v));
                 const unsigned pc = nw.popc();
                 if (writeLevels)
                     for (int i = 0; i < NW; ++i)
@@ -416,6 +444,8 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
             }
             nw.store(fout, v);
         }
+        const unsigned long long fullBits = __ballot(nowFull);
+        if (fullBits && lane_id() == 0) a.full[w] |= fullBits;  // this wave owns word w here
     }
     __syncthreads();  // the frontier words above are read by other waves below
     const unsigned running = segment_list<NW>(g, a.sv, fout, lout, l);
@@ -451,7 +481,7 @@ __global__ void __launch_bounds__(kFinalBlock) mbfs_final_kernel(GraphView g, Mb
             for (int j = 0; j < 64; ++j) lv[j * kFinalBlock + t] = 0xFF;
             unsigned long long vis = 0;
             if (live) {
-                vis = a.visited[size_t(v) * NW + i];
+                vis = vis_at<NW>(a, v)[i];
                 for (unsigned L = 0; L <= top; ++L)
                     for (unsigned long long x = a.hist[(size_t(L) * g.n + v) * NW + i]; x; x &= x - 1)
                         lv[__builtin_ctzll(x) * kFinalBlock + t] = uint8_t(L);
@@ -533,15 +563,21 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     DeviceGraph& g = *q.g;
     if (q.mbfsWords >= nw) return;
     for (void* p : {static_cast<void*>(q.mVis), static_cast<void*>(q.mFront0), static_cast<void*>(q.mFront1),
-                    static_cast<void*>(q.mNext), static_cast<void*>(q.mHist)})
+                    static_cast<void*>(q.mHist), static_cast<void*>(q.mFull)})
         q.release(p);
+    // a new layout invalidates every captured multi-source level chunk
+    for (int k : {kGraphMbfs1, kGraphMbfs2})
+        if (q.graphs[k].exec) {
+            BLEST_HIP(hipGraphExecDestroy(q.graphs[k].exec));
+            q.graphs[k].exec = nullptr;
+        }
     const size_t words = size_t(g.n) * nw;
-    q.mVis = q.alloc<unsigned long long>(words);
+    q.mVis = q.alloc<unsigned long long>(2 * words);  // visited + next, interleaved per vertex
     q.mFront0 = q.alloc<unsigned long long>(words);
     q.mFront1 = q.alloc<unsigned long long>(words);
-    q.mNext = q.alloc<unsigned long long>(words);
     q.mHistLevels = mbfs_history_levels(g.n, nw);
     q.mHist = q.alloc<unsigned long long>(words * (q.mHistLevels + 1));
+    q.mFull = q.alloc<unsigned long long>(g.W);
     if (!q.srcs) q.srcs = q.alloc<uint32_t>(128);
     if (!q.rootStats) q.rootStats = q.alloc<dev::RootStats>(128);
     q.mbfsWords = nw;
@@ -590,12 +626,12 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     BLEST_HIP(hipMemcpyAsync(q.srcs, mapped.data(), 4 * size_t(k), hipMemcpyHostToDevice, q.stream));
     dev::MbfsArgs a;
     a.levels = q.levels;
-    a.visited = q.mVis;
+    a.vn = q.mVis;
     a.front0 = q.mFront0;
     a.front1 = q.mFront1;
     a.hist = q.mHist;
     a.histLevels = q.mHistLevels;
-    a.next = q.mNext;
+    a.full = q.mFull;
     a.list0 = q.list0;
     a.list1 = q.list1;
     a.srcs = q.srcs;
