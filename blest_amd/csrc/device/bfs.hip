@@ -54,6 +54,8 @@ __device__ __forceinline__ uint32_t front_byte(const unsigned long long* f, uint
     return reinterpret_cast<const uint8_t*>(f)[r];
 }
 
+// Slice (row, hit) of a frontier column: mark the row in `next` unless already seen.
+// (Issuing all of a lane's check loads before its atomics was measured slower here.)
 __device__ __forceinline__ void expand_slice(uint32_t row, uint32_t hit, const unsigned long long* visited,
                                              unsigned long long* next) {
     if (!hit) return;

@@ -250,26 +250,42 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
                         if ((um >> b) & 1) F[b].load(fin, orr * 8 + b);
                     }
                     const uint32_t rowv[4] = {r4.x, r4.y, r4.z, r4.w};
+                    // Phased so that the four slices' loads overlap: the roots arriving
+                    // per slice (registers), then the saturated-row bits (L2), then the
+                    // row records, and only then the atomics (an atomic on a record may
+                    // alias a later slice's load, so mixing them serialises the slices).
+                    Bits<NW> acc[4];
+                    bool act[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         const uint32_t ms = (m >> (8 * s)) & 0xFF;
-                        if (!ms) continue;
-                        Bits<NW> acc;
-                        acc.zero();
+                        acc[s].zero();
 #pragma unroll
                         for (int b = 0; b < 8; ++b)
                             if ((ms >> b) & 1)
 #pragma unroll
-                                for (int i = 0; i < NW; ++i) acc.w[i] |= F[b].w[i];
-                        if (!acc.any()) continue;
-                        const uint32_t row = rowv[s];
-                        if ((a.full[row >> 6] >> (row & 63)) & 1ull) continue;  // L2-resident bitmap
-                        const unsigned long long* vr = vis_at<NW>(a, row);
-                        unsigned long long* nr = next_at<NW>(a, row);
+                                for (int i = 0; i < NW; ++i) acc[s].w[i] |= F[b].w[i];
+                        act[s] = acc[s].any();
+                    }
+                    unsigned long long fullw[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) fullw[s] = act[s] ? a.full[(act[s] ? rowv[s] : 0u) >> 6] : 0ull;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) act[s] = act[s] && !((fullw[s] >> (rowv[s] & 63)) & 1ull);
+                    unsigned long long seen[4][NW];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const unsigned long long* rec = vis_at<NW>(a, act[s] ? rowv[s] : 0u);
+#pragma unroll
+                        for (int i = 0; i < NW; ++i) seen[s][i] = act[s] ? (rec[i] | rec[NW + i]) : ~0ull;
+                    }
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        unsigned long long* nr = next_at<NW>(a, act[s] ? rowv[s] : 0u);
 #pragma unroll
                         for (int i = 0; i < NW; ++i) {
-                            const unsigned long long nv = acc.w[i] & ~(vr[i] | nr[i]);
-                            if (nv) atomicOr(nr + i, nv);
+                            const unsigned long long nv = acc[s].w[i] & ~seen[s][i];
+                            if (act[s] && nv) atomicOr(nr + i, nv);
                         }
                     }
                 }
