@@ -41,6 +41,17 @@ struct alignas(256) Ctrl {
     LevelState state[2];
 };
 
+// Streamed once per level: load the slice stream non-temporally so that it does not
+// evict the randomly accessed per-vertex state from L2 / the Infinity Cache.
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 stream_load_quad(const uint4* rows, uint32_t q) {
+    const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(rows) + q);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t stream_load_u32(const uint32_t* p, uint32_t i) {
+    return __builtin_nontemporal_load(p + i);
+}
+
 // ------------------------------------------------------------------ wave helpers
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 

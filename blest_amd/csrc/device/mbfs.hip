@@ -240,8 +240,8 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g
                 const uint32_t orr = __shfl(r, owner, kWave);
                 if (t < tq) {
                     const uint32_t q = oq0 + (t - oex);
-                    const uint4 r4 = g.rows[q];
-                    const uint32_t m = g.masks[q];
+                    const uint4 r4 = stream_load_quad(g.rows, q);
+                    const uint32_t m = stream_load_u32(g.masks, q);
                     const uint32_t um = (m | (m >> 8) | (m >> 16) | (m >> 24)) & 0xFF;
                     Bits<NW> F[8];
 #pragma unroll
@@ -474,41 +474,43 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     }
 }
 
-// Every (root, vertex) level at once, thread per vertex: the level of root j is the
-// history level whose frontier word holds bit j (kept in LDS as a byte, [root][thread]
-// so that both the scatter and the read-out are conflict-free); pairs found past the
-// history were written when found; the rest are unreached (-1).  Root-major rows, so
-// for each root the 256 threads store 256 consecutive levels.  Idempotent: enqueued
-// ahead of each completion poll, it reads only levels whose words are complete.
+// Every (root, vertex) level at once, thread per vertex.  The level of root j is the
+// history level L whose frontier word holds bit j; with L <= 16 its five bits are read
+// from five bit-planes (plane b = OR of the words of the levels with bit b set), so a
+// vertex costs its history words in and its k levels out, nothing else.  Pairs found
+// past the history were written when found; the rest are unreached (-1).  Root-major
+// rows: for each root the threads store consecutive levels.  Idempotent: enqueued ahead
+// of each completion poll, it reads only levels whose words are complete.
 constexpr unsigned kFinalBlock = 256;
 
 template <int NW>
 __global__ void __launch_bounds__(kFinalBlock) mbfs_final_kernel(GraphView g, MbfsArgs a) {
-    __shared__ uint8_t lv[64 * kFinalBlock];
     const unsigned top = min(__hip_atomic_load(&a.ctrl->histTop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                              a.histLevels);
-    const unsigned t = threadIdx.x;
-    for (uint32_t v0 = blockIdx.x * kFinalBlock; v0 < g.n; v0 += gridDim.x * kFinalBlock) {
-        const uint32_t v = v0 + t;
-        const bool live = v < g.n;
+    for (uint32_t v = blockIdx.x * kFinalBlock + threadIdx.x; v < g.n; v += gridDim.x * kFinalBlock) {
 #pragma unroll
         for (int i = 0; i < NW; ++i) {
             if (64 * i >= int(a.k)) break;
-            for (int j = 0; j < 64; ++j) lv[j * kFinalBlock + t] = 0xFF;
-            unsigned long long vis = 0;
-            if (live) {
-                vis = vis_at<NW>(a, v)[i];
-                for (unsigned L = 0; L <= top; ++L)
-                    for (unsigned long long x = a.hist[(size_t(L) * g.n + v) * NW + i]; x; x &= x - 1)
-                        lv[__builtin_ctzll(x) * kFinalBlock + t] = uint8_t(L);
+            unsigned long long plane[5] = {0, 0, 0, 0, 0}, inHist = 0;
+            for (unsigned L = 0; L <= top; ++L) {
+                const unsigned long long x = a.hist[(size_t(L) * g.n + v) * NW + i];
+                inHist |= x;
+#pragma unroll
+                for (int b = 0; b < 5; ++b)
+                    if ((L >> b) & 1u) plane[b] |= x;
             }
+            const unsigned long long vis = vis_at<NW>(a, v)[i];
             const unsigned jn = min(64u, a.k - 64u * i);
             for (unsigned j = 0; j < jn; ++j) {
-                const unsigned l = lv[j * kFinalBlock + t];
-                if (!live) continue;
                 int32_t* out = a.levels + size_t(64 * i + j) * g.n + v;
-                if (l != 0xFF) *out = int32_t(l);
-                else if (!((vis >> j) & 1ull)) *out = -1;
+                if ((inHist >> j) & 1ull) {
+                    int32_t l = 0;
+#pragma unroll
+                    for (int b = 0; b < 5; ++b) l |= int32_t((plane[b] >> j) & 1ull) << b;
+                    *out = l;
+                } else if (!((vis >> j) & 1ull)) {
+                    *out = -1;
+                }
             }
         }
     }
