@@ -153,8 +153,8 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
                     m[h] = 0;
                     if (t < tq) {
                         const uint32_t q = oq0 + (t - oex);
-                        r4[h] = g.rows[q];
-                        m[h] = g.masks[q] & (of * 0x01010101u);
+                        r4[h] = stream_load_quad(g.rows, q);
+                        m[h] = stream_load_u32(g.masks, q) & (of * 0x01010101u);
                     }
                 }
 #pragma unroll
