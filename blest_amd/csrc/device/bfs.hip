@@ -218,6 +218,7 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
     __shared__ unsigned long long tmp64[16];
     __shared__ unsigned sEx[kProducerBlock + 1];
     __shared__ uint32_t sStart[kProducerBlock];
+    __shared__ unsigned long long sWord[kProducerBlock / 8];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
     const bool bottomUp = c->state[L & 1].dir != 0;
@@ -246,28 +247,33 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
                         a.visited[w] |= word;
                     }
                     fout[w] = word;
+                    sWord[t / 8] = word;
                 }
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
             if (byte) {
                 start = g.rsetUnit[r];
                 cnt = g.rsetUnit[r + 1] - start;
-                if (!bottomUp) {
-                    news += __popc(byte);
-                    // degree loads first (all independent, unrolled), then the level stores
-                    if (canBU) {
+            }
+        } else if (!bottomUp && b == 0) {
+            sWord[t / 8] = 0ull;
+        }
+        unsigned total;
+        const unsigned ex = block_exclusive_scan(cnt, tmp, total);  // its barriers publish sWord
+        if (!bottomUp) {
+            // the new vertices' levels and degrees, thread per vertex (coalesced): the
+            // chunk's 64 words hold 8 x blockDim vertices
 #pragma unroll
-                        for (int k = 0; k < 8; ++k)
-                            if ((byte >> k) & 1u) edges += g.csrPtr[r * 8 + k + 1] - g.csrPtr[r * 8 + k];
-                    }
-#pragma unroll
-                    for (int k = 0; k < 8; ++k)
-                        if ((byte >> k) & 1u) a.levels[r * 8 + k] = int32_t(L);
+            for (int i = 0; i < 8; ++i) {
+                const unsigned li = t + i * nb;  // vertex index within the chunk
+                if ((sWord[li >> 6] >> (li & 63)) & 1ull) {
+                    const uint32_t v = c0 * 64 + li;
+                    a.levels[v] = int32_t(L);
+                    if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
+                    news += 1;
                 }
             }
         }
-        unsigned total;
-        const unsigned ex = block_exclusive_scan(cnt, tmp, total);
         sEx[t] = ex;
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
