@@ -180,11 +180,17 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
         if (u < g.n && !((vis >> lane) & 1ull)) {
             const uint32_t p0 = g.csrPtr[u], p1 = g.csrPtr[u + 1];
             deg = p1 - p0;
-            for (uint32_t p = p0; p < p1; ++p)
-                if (test_bit(fin, g.csrIdx[p])) {
+            // software-pipelined: the next neighbour's index is loaded while the current
+            // neighbour's frontier bit is tested (one dependent round trip per step)
+            uint32_t nxt = p0 < p1 ? g.csrIdx[p0] : 0u;
+            for (uint32_t p = p0; p < p1; ++p) {
+                const uint32_t cur = nxt;
+                if (p + 1 < p1) nxt = g.csrIdx[p + 1];
+                if (test_bit(fin, cur)) {
                     found = true;
                     break;
                 }
+            }
         }
         const unsigned long long nw = __ballot(found);
         if (lane == 0) {
