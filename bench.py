@@ -348,6 +348,9 @@ def main():
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
     ap.add_argument("--autotune", action="store_true", help="pick blocks_per_cu / alpha / beta with the autotuner")
+    ap.add_argument("--autotune-roots", type=int, default=32)
+    ap.add_argument("--use-graphs", type=int, default=-1,
+                    help="replay captured level chunks: 0 after a deep traversal, 1 always, 2 never (default)")
     ap.add_argument("--serve-scales", default="", help="serve: comma-separated R-MAT scales (default 18..25)")
     ap.add_argument("--serve-grids", default="", help="serve: comma-separated grid sides (default 2048,4096)")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -396,14 +399,17 @@ def main():
         opts["beta"] = a.beta
     if a.direction >= 0:
         opts["direction"] = a.direction
+    if a.use_graphs >= 0:
+        opts["use_graphs"] = a.use_graphs
     if a.mbfs:
         print(json.dumps(mbfs_bench(a, prep, opts)), flush=True)
         return
     tuned = None
     if a.autotune:
-        # launch shape and switch thresholds chosen by the native autotuner on 32 of the roots
+        # launch shape and switch thresholds chosen by the native autotuner on some of the roots
         t0 = time.time()
-        best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:32], **opts)
+        best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:a.autotune_roots],
+                                               **opts)
         for k in ("blocks_per_cu", "alpha", "beta"):
             opts[k] = best[k]
         tuned = {"blocks_per_cu": best["blocks_per_cu"], "alpha": best["alpha"], "beta": best["beta"],
@@ -545,6 +551,8 @@ def main():
                     "vertices": info["n"], "directed_edges": info["nnz"]})
     if tuned:
         cfg["autotuned"] = tuned
+    if a.use_graphs >= 0:
+        cfg["use_graphs"] = a.use_graphs
     cfg.update({"reorder": f"{w.reorder.name.lower()} window {w.window}", "roots": len(roots),
                 "slices": info["slices"], "vsets": info["vsets"]})
     line = {"metric": f"BFS GTEPS (harmonic mean, {len(roots)} roots)", "value": round(value, 3), "unit": "GTEPS",

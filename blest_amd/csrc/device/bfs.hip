@@ -240,6 +240,7 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
     for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
         const uint32_t w = c0 + t / 8, r = w * 8 + b;
         uint32_t start = 0, cnt = 0;
+        bool any = false;
         if (w < we) {
             unsigned long long word;
             if (bottomUp) {
@@ -257,6 +258,7 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
                 }
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
+            any = byte != 0;
             if (byte) {
                 start = g.rsetUnit[r];
                 cnt = g.rsetUnit[r + 1] - start;
@@ -264,6 +266,9 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
         } else if (!bottomUp && b == 0) {
             sWord[t / 8] = 0ull;
         }
+        // a chunk without any frontier vertex (most of them on a narrow level) needs no
+        // scan, levels or list: one block-wide vote instead
+        if (!__syncthreads_or(any)) continue;
         unsigned total;
         const unsigned ex = block_exclusive_scan(cnt, tmp, total);  // its barriers publish sWord
         if (!bottomUp) {
