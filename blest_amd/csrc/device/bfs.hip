@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
     }
 }
 
-__global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+__global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
     __shared__ unsigned tmp[16];
@@ -216,9 +216,9 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_expand_kernel(GraphView g,
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
 // vsets (one block scan per chunk, entries written by all threads: binary search for a
-// thread's first owner, then a forward cursor).  All of a thread's loads in a chunk are independent, so a level costs a few
+// thread's first owner, then a galloping search).  All of a thread's loads in a chunk are independent, so a level costs a few
 // memory round trips whatever the frontier size.
-__global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+__global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
@@ -289,11 +289,9 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_update_kernel(GraphView g,
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
-        // owner of the first entry by binary search, then a forward cursor: a hub rset
-        // keeps the cursor in place for hundreds of entries
         unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
         for (unsigned e = t; e < total; e += nb) {
-            while (sEx[o + 1] <= e) ++o;
+            o = owner_from(sEx, nb, o, e);
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;

@@ -18,6 +18,13 @@ namespace dev {
 constexpr int kMaxSegs = 2048;
 constexpr int kProducerBlock = 512;
 
+// A level kernel runs one workgroup per segment, and its level ends with its slowest
+// workgroup, so the default grid (4 workgroups of 512 threads per CU = 8 waves per SIMD)
+// must be resident at once: at 7 waves per SIMD (more than 64 VGPRs, or more than ~100
+// SGPRs) a CU holds 3 of them and the last quarter of the grid starts only when others
+// finish (measured: 27-33 us late on the large levels of a scale-22 BFS).
+#define BLEST_RESIDENT_GRID __attribute__((amdgpu_waves_per_eu(8)))
+
 struct SegView {
     uint32_t segs;            // producer workgroups
     uint32_t wps;             // bitmap words per segment
@@ -175,11 +182,37 @@ __device__ __forceinline__ unsigned block_owner(const unsigned* ex, unsigned nb,
     return lo;
 }
 
+// Owner of entry e >= the first entry of rset o (same ex as block_owner): gallops 1, 2,
+// 4, ... rsets ahead of o, then bisects.  A thread's next entry lies blockDim entries
+// further on: inside a hub's run of vsets the owner does not move (one LDS read), and
+// across a run of small rsets (about two vsets each on R-MAT graphs, so ~blockDim/2
+// rsets) it costs O(log) reads instead of one read per rset.
+__device__ __forceinline__ unsigned owner_from(const unsigned* ex, unsigned nb, unsigned o, unsigned e) {
+    if (ex[o + 1] > e) return o;
+    unsigned lo = o + 1, hi = nb, step = 1;  // ex[lo] <= e < ex[nb]
+    for (;;) {
+        const unsigned probe = lo + step;
+        if (probe >= nb) break;
+        if (ex[probe] > e) {
+            hi = probe;
+            break;
+        }
+        lo = probe;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (ex[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Block-wide form for a producer workgroup's whole segment [ws, we): thread = one rset
 // of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
 // are written by all threads, entry e by thread e % blockDim (coalesced); a thread finds
-// the owner of its first entry by binary search over the chunk's prefix in LDS and then
-// moves a cursor forward.  A segment holding hub rsets
+// the owner of its first entry by binary search over the chunk's prefix in LDS and of
+// each later one with owner_from.  A segment holding hub rsets
 // (hundreds of thousands of vsets) is so spread over the whole workgroup.  prep(c0) runs
 // on every thread before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds
 // blockDim + 1 entries, sStart blockDim.  Returns the segment's entry count.
@@ -206,11 +239,9 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
-        // owner of the first entry by binary search, then a forward cursor: a hub rset
-        // keeps the cursor in place for hundreds of entries
         unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
         for (unsigned e = t; e < total; e += nb) {
-            while (sEx[o + 1] <= e) ++o;
+            o = owner_from(sEx, nb, o, e);
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
