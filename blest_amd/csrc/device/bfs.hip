@@ -41,6 +41,8 @@ struct BfsArgs {
     unsigned long long* next;
     uint32_t* list0;
     uint32_t* list1;
+    uint2* chunk0;  // chunk lists (rsets of more than kChunkVsets vsets)
+    uint2* chunk1;
     Ctrl* ctrl;
     SegView sv;
     SegCounters sc;
@@ -49,6 +51,15 @@ struct BfsArgs {
     double alpha, beta;
     int levelFromCtrl;  // 1: the kernel's L argument is relative to ctrl->levelBase
 };
+
+// a[k] for a per-lane k (a select chain: the array stays in registers); 0 for k >= N.
+template <int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&a)[N], int k) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r = (i == k) ? a[i] : r;
+    return r;
+}
 
 __device__ __forceinline__ uint32_t front_byte(const unsigned long long* f, uint32_t r) {
     return reinterpret_cast<const uint8_t*>(f)[r];
@@ -76,17 +87,25 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
     }
     const uint32_t srcSeg = sw / a.sv.wps;
     for (uint32_t s = tid; s < a.sv.segs; s += nthreads) {
-        uint32_t cnt = 0, news = 0;
+        uint32_t cnt = 0, news = 0, chunks = 0;
         unsigned long long edges = 0;
         if (s == srcSeg) {
             const uint32_t r = a.src >> 3;
             const uint32_t u0 = g.rsetUnit[r], u1 = g.rsetUnit[r + 1];
-            for (uint32_t u = u0; u < u1; ++u) a.list1[a.sv.segBase[s] + (u - u0)] = u;
-            cnt = u1 - u0;
+            chunks = rset_chunks(u1 - u0);
+            if (chunks) {
+                for (uint32_t j = 0; j < chunks; ++j)
+                    a.chunk1[a.sv.chunkBase[s] + j] =
+                        make_uint2(u0 + j * kChunkVsets, min(kChunkVsets, u1 - u0 - j * kChunkVsets));
+            } else {
+                for (uint32_t u = u0; u < u1; ++u) a.list1[a.sv.segBase[s] + (u - u0)] = u;
+                cnt = u1 - u0;
+            }
             news = 1;
             edges = g.csrPtr ? (g.csrPtr[a.src + 1] - g.csrPtr[a.src]) : 0ull;
         }
         a.sc.count[a.sv.segs + s] = cnt;
+        a.sc.chunks[a.sv.segs + s] = chunks;
         a.sc.news[a.sv.segs + s] = news;
         a.sc.edges[a.sv.segs + s] = edges;
     }
@@ -117,19 +136,18 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     const unsigned sIn = L % 3, sOut = (L + 1) % 3;
     if (!bottomUp) {
+        __shared__ unsigned cpre[kMaxSegs];
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
+        const uint2* cin = (L & 1) ? a.chunk1 : a.chunk0;
         const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
+        const unsigned ctotal = load_prefix(a.sc.chunks + size_t(sIn) * segs, segs, cpre, tmp);
         const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
-        // m list entries per wave step: a short list of large vsets (the second level
-        // of an R-MAT traversal: a few thousand vsets of ~100 slices) is spread over as
-        // many waves as it has entries instead of 64 entries per wave
-        const unsigned m = min(unsigned(kWave), max(1u, (total + nwaves - 1) / nwaves));
-        for (unsigned base = wave * m; base < total; base += nwaves * m) {
-            const unsigned e = base + lane;
+        // The wave's lanes hold vsets (v valid where ok): their quads are streamed, the
+        // union of the lanes' quad runs 64 (x2) at a time.
+        auto stream = [&](uint32_t v, bool ok) {
             uint32_t q0 = 0, qc = 0, fb = 0;
-            if (lane < m && e < total) {
-                const uint32_t v = list_entry(lin, a.sv, pre, e);
+            if (ok) {
                 q0 = g.unitQuad[v];
                 qc = g.unitQuad[v + 1] - q0;
                 fb = front_byte(fin, g.unitRset[v]);
@@ -165,42 +183,152 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                     expand_slice(r4[h].w, (m[h] >> 24) & 0xFF, a.visited, a.next);
                 }
             }
+        };
+        // m list entries (or chunk slots) per wave step, from the level's whole work
+        // (list entries + kChunkVsets slots per chunk): a short list of large vsets (the second
+        // level of an R-MAT traversal: a few thousand vsets of ~100 slices) is spread over
+        // as many waves as it has entries instead of 64 entries per wave, and a long one
+        // takes as few steps per wave as 64 vsets per step allow
+        const unsigned cslots = ctotal * kChunkVsets;
+        const unsigned target = (total + cslots + nwaves - 1) / nwaves;
+        const unsigned m = min(unsigned(kWave), max(1u, target));
+        for (unsigned base = wave * m; base < total; base += nwaves * m) {
+            const unsigned e = base + lane;
+            const bool ok = lane < m && e < total;
+            stream(ok ? list_entry(lin, a.sv, pre, e) : 0u, ok);
+        }
+        // then the chunks (<= kChunkVsets consecutive vsets of one rset), seen as
+        // kChunkVsets slots each and
+        // enumerated offset-major (slot 0 of every chunk, then slot 1, ...), m slots per
+        // wave step like the list: a step's lanes hold vsets of different chunks, so their
+        // rows (and `next` atomics) are spread as a list step's are, instead of 64
+        // consecutive vsets of one hub whose rows crowd into the same bitmap words; the
+        // round robin over the waves continues after the list's steps
+        const unsigned lsteps = (total + m - 1) / m;
+        for (unsigned base = ((wave + nwaves - lsteps % nwaves) % nwaves) * m; base < cslots; base += nwaves * m) {
+            const unsigned e = base + lane;
+            bool ok = lane < m && e < cslots;
+            uint32_t v = 0;
+            if (ok) {
+                const uint2 ch = chunk_entry(cin, a.sv, cpre, e % ctotal);
+                const uint32_t off = e / ctotal;
+                ok = off < ch.y;
+                v = ch.x + off;
+            }
+            stream(v, ok);
         }
         return;
     }
-    // bottom-up: this workgroup's segment, wave per word, lane per vertex
+    // bottom-up over this workgroup's segment.  Wave wib takes the words ws + wib,
+    // ws + wib + wpb, ... in groups of kBuGroup; lane l owns vertex 64w + l of every
+    // word of the group and scans the in-neighbours of its open vertices one after
+    // another, 1, 2, then kBuBatch entries per step (a step's loads issued together;
+    // most vertices of a bottom-up level hit at their first entry), independently of
+    // the other lanes: a wave runs as long as its busiest lane, not as the sum over its
+    // words of each word's slowest lane.  A scan still without a hit after kBuBudget
+    // entries is handed, from where it stopped, to a pass in which a whole wave scans
+    // the rest 64 entries at a time, so a long unsuccessful scan does not hold one lane
+    // for the whole level.  (A fully wave-cooperative scan, lanes dealt out over the
+    // word's open vertices every step, measured slower: its per-step shuffles cost as
+    // much as a memory round trip.)
+    constexpr int kBuGroup = 4, kBuBatch = 4;
+    constexpr uint32_t kBuBudget = 32;
+    constexpr unsigned kHeavyCap = 1024;
+    __shared__ uint32_t heavyU[kHeavyCap], heavyP[kHeavyCap];
+    __shared__ unsigned heavyN;
+    if (threadIdx.x == 0) heavyN = 0;
+    __syncthreads();
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
-    const unsigned wib = threadIdx.x / kWave, wpb = blockDim.x / kWave;
+    // wave-uniform (readfirstlane): the group's word indices live in scalar registers
+    const unsigned wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), wpb = blockDim.x / kWave;
     unsigned long long news = 0, edges = 0;
-    for (uint32_t w = ws + wib; w < we; w += wpb) {
-        const unsigned long long vis = a.visited[w];
-        const uint32_t u = w * 64 + lane;
-        bool found = false;
-        uint32_t deg = 0;
-        if (u < g.n && !((vis >> lane) & 1ull)) {
-            const uint32_t p0 = g.csrPtr[u], p1 = g.csrPtr[u + 1];
-            deg = p1 - p0;
-            // software-pipelined: the next neighbour's index is loaded while the current
-            // neighbour's frontier bit is tested (one dependent round trip per step)
-            uint32_t nxt = p0 < p1 ? g.csrIdx[p0] : 0u;
-            for (uint32_t p = p0; p < p1; ++p) {
-                const uint32_t cur = nxt;
-                if (p + 1 < p1) nxt = g.csrIdx[p + 1];
-                if (test_bit(fin, cur)) {
-                    found = true;
-                    break;
+    for (uint32_t w0 = ws + wib; w0 < we; w0 += wpb * kBuGroup) {
+        uint32_t pb[kBuGroup], pe[kBuGroup];
+        unsigned cand = 0;  // bit k: the vertex of word k is open and has in-neighbours
+#pragma unroll
+        for (int k = 0; k < kBuGroup; ++k) {
+            const uint32_t w = w0 + k * wpb;
+            const unsigned long long vis = w < we ? a.visited[w] : ~0ull;
+            const uint32_t u = w * 64 + lane;
+            const bool open = w < we && u < g.n && !((vis >> lane) & 1ull);
+            pb[k] = open ? g.csrPtr[u] : 0u;
+            pe[k] = open ? g.csrPtr[u + 1] : 0u;
+            if (pe[k] > pb[k]) cand |= 1u << k;
+        }
+        unsigned found = 0;
+        int k = cand ? __builtin_ctz(cand) : kBuGroup;
+        uint32_t p = pick(pb, k), p1 = pick(pe, k);
+        uint32_t budget = p + kBuBudget;
+        unsigned step = 1;
+        while (k < kBuGroup) {  // p < p1 here
+            const uint32_t lim = min(p + step, p1);
+            uint32_t nb[kBuBatch];
+            unsigned long long fw[kBuBatch];
+#pragma unroll
+            for (int i = 0; i < kBuBatch; ++i) nb[i] = p + i < lim ? g.csrIdx[p + i] : 0u;
+#pragma unroll
+            for (int i = 0; i < kBuBatch; ++i) fw[i] = p + i < lim ? fin[nb[i] >> 6] : 0ull;
+            bool hit = false;
+#pragma unroll
+            for (int i = 0; i < kBuBatch; ++i) hit |= (fw[i] >> (nb[i] & 63)) & 1ull;
+            p = lim;
+            step = min(2 * step, unsigned(kBuBatch));
+            bool next = hit || p >= p1;
+            if (!next && p >= budget) {
+                const unsigned slot = atomicAdd(&heavyN, 1u);
+                if (slot < kHeavyCap) {
+                    heavyU[slot] = (w0 + k * wpb) * 64 + lane;
+                    heavyP[slot] = p;
+                    next = true;
+                } else {
+                    budget = p1;  // queue full: this lane finishes the scan itself
+                }
+            }
+            if (next) {
+                if (hit) found |= 1u << k;
+                cand &= ~(1u << k);
+                k = cand ? __builtin_ctz(cand) : kBuGroup;
+                p = pick(pb, k);
+                p1 = pick(pe, k);
+                budget = p + kBuBudget;
+                step = 1;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kBuGroup; ++k) {
+            const uint32_t w = w0 + k * wpb;
+            const bool f = (found >> k) & 1u;
+            const unsigned long long nw = __ballot(f);
+            if (w < we) {
+                if (lane == 0) {
+                    if (nw) atomicOr(&a.visited[w], nw);  // result unused: no round trip
+                    fout[w] = nw;
+                }
+                if (f) {
+                    a.levels[w * 64 + lane] = int32_t(L);
+                    news += 1;
+                    edges += pe[k] - pb[k];
                 }
             }
         }
-        const unsigned long long nw = __ballot(found);
-        if (lane == 0) {
-            if (nw) a.visited[w] = vis | nw;
-            fout[w] = nw;
+    }
+    __syncthreads();  // the queue is complete and every word's first store precedes the ORs below
+    const unsigned nh = min(heavyN, kHeavyCap);
+    for (unsigned h = wib; h < nh; h += wpb) {
+        const uint32_t u = heavyU[h];
+        const uint32_t p0 = g.csrPtr[u], p1 = g.csrPtr[u + 1];
+        bool hit = false;
+        for (uint32_t q0 = heavyP[h]; q0 < p1 && !hit; q0 += kWave) {  // wave-uniform
+            const uint32_t q = q0 + lane;
+            hit = __ballot(q < p1 && test_bit(fin, g.csrIdx[min(q, p1 - 1)])) != 0ull;
         }
-        if (found) {
+        if (hit && lane == 0) {
+            const unsigned long long bit = 1ull << (u & 63);
+            atomicOr(&a.visited[u >> 6], bit);
+            atomicOr(&fout[u >> 6], bit);
             a.levels[u] = int32_t(L);
             news += 1;
-            edges += deg;
+            edges += p1 - p0;
         }
     }
     news = block_sum_u64(news, tmp64);
@@ -230,16 +358,17 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     const bool bottomUp = c->state[L & 1].dir != 0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     uint32_t* out = ((L & 1) ? a.list0 : a.list1) + a.sv.segBase[blockIdx.x];
+    uint2* cout = ((L & 1) ? a.chunk0 : a.chunk1) + a.sv.chunkBase[blockIdx.x];
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
     const unsigned t = threadIdx.x, b = t & 7, nb = blockDim.x;
     const bool canBU = g.csrPtr != nullptr;
     unsigned long long news = 0, edges = 0;
-    unsigned written = 0;
+    unsigned written = 0, cwritten = 0;
     for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
         const uint32_t w = c0 + t / 8, r = w * 8 + b;
-        uint32_t start = 0, cnt = 0;
+        uint32_t start = 0, cnt = 0, chunks = 0;
         bool any = false;
         if (w < we) {
             unsigned long long word;
@@ -262,6 +391,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             if (byte) {
                 start = g.rsetUnit[r];
                 cnt = g.rsetUnit[r + 1] - start;
+                chunks = rset_chunks(cnt);  // a hub's rset: chunk entries, not vsets
             }
         } else if (!bottomUp && b == 0) {
             sWord[t / 8] = 0ull;
@@ -270,7 +400,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         // scan, levels or list: one block-wide vote instead
         if (!__syncthreads_or(any)) continue;
         unsigned total;
-        const unsigned ex = block_exclusive_scan(cnt, tmp, total);  // its barriers publish sWord
+        const unsigned ex = block_exclusive_scan(chunks ? 0u : cnt, tmp, total);  // its barriers publish sWord
         if (!bottomUp) {
             // the new vertices' levels and degrees, thread per vertex (coalesced): the
             // chunk's 64 words hold 8 x blockDim vertices
@@ -285,7 +415,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
                 }
             }
         }
-        sEx[t] = ex;
+        sEx[t] = ex;  // rsets listed as chunks own no entries here
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
         __syncthreads();
@@ -295,7 +425,11 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
-        __syncthreads();  // the next chunk reuses sEx / sStart
+        unsigned ctotal;
+        const unsigned cex = block_exclusive_scan(chunks, tmp, ctotal);  // also orders sEx / sStart reuse
+        for (uint32_t j = 0; j < chunks; ++j)
+            cout[cwritten + cex + j] = make_uint2(start + j * kChunkVsets, min(kChunkVsets, cnt - j * kChunkVsets));
+        cwritten += ctotal;
     }
     if (!bottomUp) {
         news = block_sum_u64(news, tmp64);
@@ -303,6 +437,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     }
     if (t == 0) {
         a.sc.count[size_t(sOut) * segs + blockIdx.x] = written;
+        a.sc.chunks[size_t(sOut) * segs + blockIdx.x] = cwritten;
         if (!bottomUp) {
             a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(news);
             a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;
@@ -339,6 +474,8 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     a.next = q.next;
     a.list0 = q.list0;
     a.list1 = q.list1;
+    a.chunk0 = q.chunk0;
+    a.chunk1 = q.chunk1;
     a.ctrl = q.ctrl;
     a.sv = g.seg_view();
     a.sc = q.seg_counters();

@@ -27,9 +27,9 @@ Query::~Query() {
 
 dim3 level_grid(const DeviceGraph& g) { return dim3(g.segs); }
 
-dev::SegView DeviceGraph::seg_view() const { return dev::SegView{segs, wps, segBase}; }
+dev::SegView DeviceGraph::seg_view() const { return dev::SegView{segs, wps, segBase, chunkBase}; }
 
-dev::SegCounters Query::seg_counters() const { return dev::SegCounters{segCount, segNew, segEdges}; }
+dev::SegCounters Query::seg_counters() const { return dev::SegCounters{segCount, segNew, segEdges, segChunks}; }
 
 namespace dev {
 // Closes a replayed chunk of `count` levels: the next chunk starts after it.
@@ -181,6 +181,20 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
                 }
                 g.segBase = g.alloc<uint32_t>(sb.size());
                 BLEST_HIP(hipMemcpy(g.segBase, sb.data(), 4 * sb.size(), hipMemcpyHostToDevice));
+                // chunk slots: segment s may emit the chunks of its own rsets only
+                std::vector<uint32_t> cb(size_t(g.segs) + 1, 0);
+                uint64_t total = 0;
+                for (uint32_t s = 0; s < g.segs; ++s) {
+                    cb[s] = uint32_t(total);
+                    const uint64_t r0 = std::min<uint64_t>(uint64_t(s) * g.wps * 8, g.R);
+                    const uint64_t r1 = std::min<uint64_t>(uint64_t(s + 1) * g.wps * 8, g.R);
+                    for (uint64_t r = r0; r < r1; ++r) total += dev::rset_chunks(b.realPtrs[r + 1] - b.realPtrs[r]);
+                }
+                if (total > 0xFFFFFFFFull) throw Error(BLEST_ERR_UNSUPPORTED, "more than 2^32 list chunks");
+                cb[g.segs] = uint32_t(total);
+                g.chunks = total;
+                g.chunkBase = g.alloc<uint32_t>(cb.size());
+                BLEST_HIP(hipMemcpy(g.chunkBase, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice));
             }
             // Device slice stream: the real slices of each vset packed four to a quad.  The
             // reference layout's padding slots (row 0xFFFFFFFF with an empty mask; a
@@ -325,6 +339,9 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
             q.next = q.alloc<unsigned long long>(g.W);
             q.list0 = q.alloc<uint32_t>(g.V);
             q.list1 = q.alloc<uint32_t>(g.V);
+            q.chunk0 = q.alloc<uint2>(g.chunks);
+            q.chunk1 = q.alloc<uint2>(g.chunks);
+            q.segChunks = q.alloc<uint32_t>(3 * size_t(g.segs));
             q.segCount = q.alloc<uint32_t>(3 * size_t(g.segs));
             q.segNew = q.alloc<uint32_t>(3 * size_t(g.segs));
             q.segEdges = q.alloc<unsigned long long>(3 * size_t(g.segs));
@@ -374,7 +391,7 @@ blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_so
         uint64_t b = sizeof(dev::Ctrl);
         const uint64_t levels = 4ull * g.n * max_sources;
         b += levels * (g.hasPerm ? 2 : 1);
-        b += 4ull * 8ull * g.W + 2ull * 4ull * g.V + 3ull * 16ull * g.segs;
+        b += 4ull * 8ull * g.W + 2ull * 4ull * g.V + 2ull * 8ull * g.chunks + 3ull * 20ull * g.segs;
         if (max_sources > 1) {
             const uint32_t nw = max_sources > 64 ? 2 : 1;
             b += (4ull + mbfs_history_levels(g.n, nw) + 1) * 8ull * g.n * nw + 4ull * 128 + 16ull * 128;

@@ -52,6 +52,8 @@ struct DeviceGraph {
     uint32_t blockSize = 256, blocksPerCu = 4, gridBlocks = 0;
     uint32_t segs = 1, wps = 1;    // producer segments and bitmap words per segment
     uint32_t* segBase = nullptr;   // segs + 1 first list slots
+    uint32_t* chunkBase = nullptr; // segs + 1 first chunk slots (rsets of > kChunkVsets vsets)
+    uint64_t chunks = 0;           // chunk slots in all
     dev::SegView seg_view() const;
     blest_device_opts opts{};
     uint32_t n = 0, R = 0, V = 0, W = 0;
@@ -117,7 +119,9 @@ struct Query {
     int32_t* levels = nullptr;  // n * maxSources
     unsigned long long *visited = nullptr, *front0 = nullptr, *front1 = nullptr, *next = nullptr;
     uint32_t *list0 = nullptr, *list1 = nullptr;
+    uint2 *chunk0 = nullptr, *chunk1 = nullptr;        // chunk lists (single-source BFS)
     uint32_t *segCount = nullptr, *segNew = nullptr;  // [3][segs]
+    uint32_t* segChunks = nullptr;                     // [3][segs]
     unsigned long long* segEdges = nullptr;            // [3][segs]
     dev::SegCounters seg_counters() const;
     int32_t* levelsOut = nullptr;  // original-order copy (when the graph carries a permutation)

@@ -25,10 +25,20 @@ constexpr int kProducerBlock = 512;
 // finish (measured: 27-33 us late on the large levels of a scale-22 BFS).
 #define BLEST_RESIDENT_GRID __attribute__((amdgpu_waves_per_eu(8)))
 
+// An active rset with more than kChunkVsets vsets (a hub's: up to thousands of vsets,
+// and an ordering such as Gorder puts hubs next to each other, so one segment can hold
+// 10^5 vsets of them against ~10^3 on average) is not listed vset by vset but as chunk
+// entries (first vset, count <= kChunkVsets) in a second segmented list: its producer
+// writes up to 16 times fewer entries (a segment's list is at most 16 per rset), and
+// chunks hold >= kChunkVsets / 2 vsets on average.  The BFS producer and consumer use
+// chunks; the multi-source and label-propagation lists stay per vset.
+constexpr uint32_t kChunkVsets = 16;
+
 struct SegView {
-    uint32_t segs;            // producer workgroups
-    uint32_t wps;             // bitmap words per segment
-    const uint32_t* segBase;  // segs + 1: first list slot of each segment
+    uint32_t segs;              // producer workgroups
+    uint32_t wps;               // bitmap words per segment
+    const uint32_t* segBase;    // segs + 1: first list slot of each segment
+    const uint32_t* chunkBase;  // segs + 1: first chunk slot of each segment
 };
 
 // Per-slot per-segment counters (slot = level % 3).
@@ -36,7 +46,13 @@ struct SegCounters {
     uint32_t* count;          // [3][segs] list entries
     uint32_t* news;           // [3][segs] vertices (or vertex-root pairs) discovered
     unsigned long long* edges;  // [3][segs] their degree sum
+    uint32_t* chunks;         // [3][segs] chunk entries
 };
+
+// Chunk entries an rset with cnt vsets contributes (0 when it is listed per vset).
+__host__ __device__ __forceinline__ uint32_t rset_chunks(uint32_t cnt) {
+    return cnt > kChunkVsets ? (cnt + kChunkVsets - 1) / kChunkVsets : 0u;
+}
 
 // Exclusive scan over the block (blockDim.x <= 1024, multiple of 64); returns the
 // thread's exclusive prefix and the block total.  `tmp` holds >= 16 entries.
@@ -115,6 +131,14 @@ __device__ __forceinline__ uint32_t list_entry(const uint32_t* list, const SegVi
     const uint32_t s = find_segment(pre, sv.segs, e);
     const unsigned before = s ? pre[s - 1] : 0u;
     return list[sv.segBase[s] + (e - before)];
+}
+
+// Chunk entry with global index c (cpre = inclusive prefix of the chunk counts).
+__device__ __forceinline__ uint2 chunk_entry(const uint2* chunks, const SegView& sv, const unsigned* cpre,
+                                             unsigned c) {
+    const uint32_t s = find_segment(cpre, sv.segs, c);
+    const unsigned before = s ? cpre[s - 1] : 0u;
+    return chunks[sv.chunkBase[s] + (c - before)];
 }
 
 // Number of vsets of the rsets with a nonzero byte in word w's bits.
