@@ -10,8 +10,9 @@ Per-root GTEPS = m_c / t, with m_c = half the degree sum of the reached vertices
 HIP events on the query's stream; the reported value is the harmonic mean over all
 (step, root) launches.  The line also carries:
   roofline      algorithmic bytes per launch (the reference formulation's B_td / B_bu,
-                replayed by the oracle on the exact direction schedule each launch ran)
-                / mean launch time, against 8 TB/s;
+                computed by reference_level_bytes from a sample of launches' own GPU
+                levels and direction schedules) / those launches' mean time, against
+                8 TB/s;
   cpu_baseline  the oracle's level-synchronous BVSS BFS (the reference algorithm,
                 its own direction rule) on all host cores over a bounded root sample.
 """
@@ -52,6 +53,55 @@ def harmonic(values):
 
 
 SERVE_RATES = (2000, 5000, 10000, 20000, 50000)
+
+
+def reference_level_bytes(n, col_ptrs, rows, bv, levels, bu_mask, entry_row=None):
+    """Algorithmic bytes per level of one single-source BFS in the reference formulation
+    (BFS/BVSSBFSKernel.cuh:96-276; the fixed byte model of the measurement plan), from the
+    traversal's own output: levels (reordered IDs, -1 unreached) and the bottom-up mask
+    the launch ran.  W = 32-bit bitmap words, s_v = slices of vset v in the BVSS layout,
+    e_u = in-neighbours scanned in sorted order up to the first frontier hit (deg if none):
+      top-down  B = sum_{v in VF} (5 s_v + 25) + 12 W + 4 |F'| + 4 |VF'|
+      bottom-up B = 4 W + n/8 + sum_{u unvisited} (8 + 4 e_u) + 12 W + 4 |F'|
+    Levels 1 .. max+1 (the last one finds nothing, as the traversal's last launch).
+    entry_row (optional, reused across roots): the CSR row of every entry."""
+    lv = np.asarray(levels, dtype=np.int64)
+    cp = np.asarray(col_ptrs, dtype=np.int64)
+    W = (n + 31) // 32
+    rp = bv["real_ptrs"].astype(np.int64)
+    ssp = bv["slice_set_ptrs"].astype(np.int64)
+    R = len(rp) - 1
+    rvs = np.diff(rp)                          # vsets per rset
+    rsl = ssp[rp[1:]] - ssp[rp[:-1]]           # slices per rset
+    top = int(lv.max()) if (lv >= 0).any() else 0
+    reached = lv >= 0
+    # per level: distinct rsets holding a vertex of that level
+    keys = np.unique(lv[reached] * R + (np.nonzero(reached)[0] >> 3))
+    kl, kr = keys // R, keys % R
+    lvs_vsets = np.bincount(kl, weights=rvs[kr], minlength=top + 2)
+    lvs_slices = np.bincount(kl, weights=rsl[kr], minlength=top + 2)
+    new_count = np.bincount(lv[reached], minlength=top + 2)
+    out = []
+    for L in range(1, top + 2):
+        bu = L <= 64 and (bu_mask >> (L - 1)) & 1
+        newv = int(new_count[L]) if L <= top else 0
+        cnt2 = float(lvs_vsets[L]) if L <= top else 0.0
+        if not bu:
+            b = 5 * lvs_slices[L - 1] + 25 * lvs_vsets[L - 1] + 12 * W + 4 * newv + 4 * cnt2
+        else:
+            if entry_row is None:
+                entry_row = np.repeat(np.arange(n, dtype=np.int64), np.diff(cp))
+            open_ = (lv < 0) | (lv >= L)
+            unv = int(open_.sum())
+            hits = np.flatnonzero(open_[entry_row] & (lv[rows] == L - 1))
+            hr = entry_row[hits]
+            first = np.ones(len(hr), dtype=bool)
+            first[1:] = hr[1:] != hr[:-1]
+            # an open row with a hit scans up to its first hit; every other open row its whole list
+            scanned = int(np.diff(cp)[open_].sum()) - int((cp[hr[first] + 1] - hits[first] - 1).sum())
+            b = 4 * W + (n + 7) // 8 + 8 * unv + 4 * scanned + 12 * W + 4 * newv
+        out.append(int(b))
+    return out
 
 
 def serve_bench(a) -> dict:
@@ -341,7 +391,7 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-cc", action="store_true", help="skip the connected-components timing")
     ap.add_argument("--roofline-roots", type=int, default=-1,
-                    help="roots replayed for the byte model (-1: all up to 2^23 vertices, else 8; 0: all)")
+                    help="roots whose GPU levels feed the byte model (-1: 8; 0: all)")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--alpha", type=float, default=0.0)
@@ -450,44 +500,42 @@ def main():
     roofline = None
     cpu = None
     if a.roofline_roots < 0:
-        a.roofline_roots = 0 if prep.csc.n <= (1 << 23) else 8
+        a.roofline_roots = 8
+    bv = None
     if not a.no_roofline or not a.no_cpu_baseline:
         import ctypes as C
-
-        import oracle
         bv = prep.bvss.arrays()
         cp, rows = prep.csc.arrays()  # symmetric: the CSR of the bottom-up pass is the CSC
-        threads = oracle.max_threads()
     if not a.no_roofline:
-        # Algorithmic bytes per launch: the reference formulation's per-level bytes on the
-        # direction schedule the launch actually ran (one replay per root; deterministic).
+        # Algorithmic bytes per launch: the reference formulation's per-level bytes,
+        # computed from each replayed root's own GPU levels and the direction schedule it
+        # ran (reference_level_bytes).  Per direction: each replayed root traced once (a
+        # HIP event around every kernel).
         t = time.time()
-        per_root, per_level = {}, {}
         replay_roots = roots if a.roofline_roots <= 0 else roots[:a.roofline_roots]
-        for r in replay_roots:
-            mask = next(s[3] for s in samples if s[0] == r)
-            sched = [(mask >> i) & 1 for i in range(64)]
-            _, _, levels = oracle.bvss_bfs(prep.csc.n, bv, cp, rows, int(prep.inv_perm[r]), schedule=sched,
-                                           threads=threads)
-            per_root[r] = sum(l["bytes"] for l in levels)
-            per_level[r] = levels
-        used = [s for s in samples if s[0] in per_root]
-        total_bytes = sum(per_root[s[0]] for s in used)
-        total_s = sum(s[1] for s in used) * 1e-3
-        achieved = total_bytes / total_s / 1e9
-        # per direction: each replayed root traced once (a HIP event around every kernel)
+        entry_row = np.repeat(np.arange(prep.csc.n, dtype=np.int64), np.diff(cp.astype(np.int64)))
         tr = (B._lib.LevelTrace * 4096)()
         cnt = C.c_uint32()
         st = B._lib.BfsStats()
+        lv = np.empty(prep.csc.n, dtype=np.int32)
+        lvr = np.empty_like(lv)
+        per_root, per_level = {}, {}
         by_dir = {0: [0, 0.0], 1: [0, 0.0]}  # bytes, seconds
         for r in replay_roots:
-            B.check(lib.blest_bfs_trace(q, r, None, C.byref(st), tr, 4096, C.byref(cnt)))
-            for i, lv in enumerate(per_level[r]):
+            B.check(lib.blest_bfs_trace(q, r, lv.ctypes.data_as(B._lib.I32P), C.byref(st), tr, 4096, C.byref(cnt)))
+            lvr[prep.inv_perm] = lv  # levels by reordered ID
+            per_level[r] = reference_level_bytes(prep.csc.n, cp, rows, bv, lvr, int(st.bottom_up_mask), entry_row)
+            per_root[r] = sum(per_level[r])
+            for i, b in enumerate(per_level[r]):
                 if i >= cnt.value:
                     break
                 d = int(tr[i].bottom_up)
-                by_dir[d][0] += lv["bytes"]
+                by_dir[d][0] += b
                 by_dir[d][1] += (tr[i].expand_ms + tr[i].update_ms) * 1e-3
+        used = [s_ for s_ in samples if s_[0] in per_root]
+        total_bytes = sum(per_root[s_[0]] for s_ in used)
+        total_s = sum(s_[1] for s_ in used) * 1e-3
+        achieved = total_bytes / total_s / 1e9
         directions = {}
         for d, name in ((0, "top_down"), (1, "bottom_up")):
             if by_dir[d][1] > 0:
@@ -498,15 +546,18 @@ def main():
         tf = ROOT / "profiles" / f"bfs_kernel_{a.workload}_traffic.json"
         if tf.exists():
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        used_ms = float(np.mean([s_[1] for s_ in used]))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": "bfs_kernel", "algorithmic_bytes_per_launch": int(total_bytes / len(used)),
-                    "mean_launch_ms": round(mean_ms, 4), "replayed_roots": len(replay_roots),
+                    "mean_launch_ms": round(used_ms, 4), "replayed_roots": len(replay_roots),
                     "by_direction": directions}
-        log(f"[bench] byte model replayed in {time.time() - t:.1f}s: {achieved:.1f} GB/s")
+        log(f"[bench] byte model over {len(replay_roots)} roots in {time.time() - t:.1f}s: {achieved:.1f} GB/s")
     if not a.no_cpu_baseline:
         # Reference CPU path: level-synchronous BVSS BFS with the reference's switching
         # rule, all host cores, over as many roots as fit the time budget.
+        import oracle
+        threads = oracle.max_threads()
         t = time.time()
         cg, cr = [], []
         # BVSSBFSKernel::hostCode picks the switching kernel only for lazy (update

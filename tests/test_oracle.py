@@ -262,3 +262,24 @@ def test_social_closed_forms():
     assert sdd(np.concatenate([deg, np.ones(50 * len(deg), dtype=int)]))  # only the fit fires
     assert not sdd([])
     assert not sdd([0, 0, 0])
+
+
+def test_bench_byte_model_matches_oracle_replay(oracle, blest):
+    """bench.reference_level_bytes (numpy, from a traversal's own levels and direction
+    mask) equals the oracle's level replay byte counts, level by level, for top-down,
+    bottom-up and mixed schedules on social, grid and sparse graphs."""
+    import bench
+    rng = np.random.default_rng(0)
+    for make, reorder in [(lambda: blest.CSC.rmat(11, 16, seed=1), blest.Reorder.JACCARD),
+                          (lambda: blest.CSC.grid(48, 0.9, 0.01, 2), blest.Reorder.RCM),
+                          (lambda: blest.CSC.erdos_renyi(3001, 3.0, 4), None)]:
+        g = make()
+        if reorder is not None:
+            g.reorder(reorder, window=1024)
+        bv = blest.BVSS(8, 4).construct_from_csc(g, full_padding=not g.is_social_network()).arrays()
+        cp, rows = g.arrays()
+        for src in g.pick_roots(2, seed=2):
+            for mask in (0, 0b110, 0b11110, int(rng.integers(0, 1 << 20))):
+                sched = [(mask >> i) & 1 for i in range(64)]
+                lv, _, per = oracle.bvss_bfs(g.n, bv, cp, rows, int(src), schedule=sched, threads=2)
+                assert bench.reference_level_bytes(g.n, cp, rows, bv, lv, mask) == [p["bytes"] for p in per]
