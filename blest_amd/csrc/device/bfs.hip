@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         if (!bottomUp) {
             // the new vertices' levels and degrees, thread per vertex (coalesced): the
             // chunk's 64 words hold 8 x blockDim vertices
-#pragma unroll
+#pragma unroll 4  // (8 in flight spilled past the 64 VGPRs of a resident grid)
             for (int i = 0; i < 8; ++i) {
                 const unsigned li = t + i * nb;  // vertex index within the chunk
                 if ((sWord[li >> 6] >> (li & 63)) & 1ull) {

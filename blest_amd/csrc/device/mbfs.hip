@@ -191,8 +191,18 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphVi
     if (threadIdx.x == 0) a.sc.count[a.sv.segs + blockIdx.x] = cnt;
 }
 
+// Launched with mbfs_expand_block<NW>() threads per workgroup, one per segment: the grid
+// must be resident at once (the level ends with its slowest workgroup).  64 roots fit 64
+// VGPRs (8 waves per SIMD: 4 workgroups of 512 per CU); 128 roots need ~86 (5 waves per
+// SIMD), so they run as workgroups of 256 (5 per CU), every workgroup resident.
 template <int NW>
-__global__ void __launch_bounds__(kProducerBlock) mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
+constexpr unsigned mbfs_expand_block() {
+    return NW == 1 ? kProducerBlock : kProducerBlock / 2;
+}
+
+template <int NW>
+__global__ void __launch_bounds__(kProducerBlock) __attribute__((amdgpu_waves_per_eu(NW == 1 ? 8 : 5)))
+mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
     __shared__ unsigned tmp[16];
@@ -605,7 +615,7 @@ template <int NW>
 static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     DeviceGraph& g = *q.g;
     const GraphView gv = g.view();
-    const dim3 grid = level_grid(g), block(dev::kProducerBlock);
+    const dim3 grid = level_grid(g), block(dev::kProducerBlock), eblock(dev::mbfs_expand_block<NW>());
     hipLaunchKernelGGL(dev::mbfs_init_kernel<NW>, grid, block, 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_seed_kernel<NW>, dim3(2), dim3(64), 0, q.stream, gv, a);
     hipLaunchKernelGGL(dev::mbfs_first_list_kernel<NW>, grid, block, 0, q.stream, gv, a);
@@ -614,7 +624,7 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     run_levels(
         q,
         [&](unsigned L) {
-            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, eblock, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
         },
         [&] {
@@ -623,7 +633,7 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
         },
         &q.graphs[NW == 1 ? kGraphMbfs1 : kGraphMbfs2],
         [&](unsigned k) {
-            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, eblock, 0, q.stream, gv, ar, k);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
         },
         launch_key(a.alpha, a.beta, a.direction, a.k));
