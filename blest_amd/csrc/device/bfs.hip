@@ -118,7 +118,6 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
 __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
-    __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
     DirectionParams dp;
     dp.direction = a.direction;
@@ -130,17 +129,16 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     Ctrl* c = a.ctrl;
     const uint32_t segs = a.sv.segs;
     bool bottomUp = false;
-    if (!level_begin(c, a.sc, segs, L, dp, bottomUp)) return;
+    __shared__ unsigned cpre[kMaxSegs];
+    unsigned total = 0, ctotal = 0;  // list entries / chunks of the level (top-down)
+    if (!level_begin(c, a.sc, segs, L, dp, bottomUp, pre, &total, cpre, &ctotal)) return;
     const unsigned lane = lane_id();
     const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
-    const unsigned sIn = L % 3, sOut = (L + 1) % 3;
+    const unsigned sOut = (L + 1) % 3;
     if (!bottomUp) {
-        __shared__ unsigned cpre[kMaxSegs];
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
         const uint2* cin = (L & 1) ? a.chunk1 : a.chunk0;
-        const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
-        const unsigned ctotal = load_prefix(a.sc.chunks + size_t(sIn) * segs, segs, cpre, tmp);
         const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
         // The wave's lanes hold vsets (v valid where ok): their quads are streamed, the
@@ -354,15 +352,22 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     __shared__ uint32_t sStart[kProducerBlock];
     __shared__ unsigned long long sWord[kProducerBlock / 8];
     Ctrl* c = a.ctrl;
-    if (ld_relaxed(&c->done)) return;
-    const bool bottomUp = c->state[L & 1].dir != 0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
-    uint32_t* out = ((L & 1) ? a.list0 : a.list1) + a.sv.segBase[blockIdx.x];
-    uint2* cout = ((L & 1) ? a.chunk0 : a.chunk1) + a.sv.chunkBase[blockIdx.x];
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
     const unsigned t = threadIdx.x, b = t & 7, nb = blockDim.x;
+    // one batch of independent loads: the flags, the segment's list bases and both
+    // candidate words of the first chunk (next for top-down, fout for bottom-up)
+    const unsigned done = ld_relaxed(&c->done);
+    const unsigned dir = c->state[L & 1].dir;
+    const uint32_t listBase = a.sv.segBase[blockIdx.x], chunkBase = a.sv.chunkBase[blockIdx.x];
+    const uint32_t w1 = min(ws + t / 8, g.W - 1);
+    const unsigned long long firstNext = a.next[w1], firstFout = fout[w1];
+    if (done) return;
+    const bool bottomUp = dir != 0;
+    uint32_t* out = ((L & 1) ? a.list0 : a.list1) + listBase;
+    uint2* cout = ((L & 1) ? a.chunk0 : a.chunk1) + chunkBase;
     const bool canBU = g.csrPtr != nullptr;
     unsigned long long news = 0, edges = 0;
     unsigned written = 0, cwritten = 0;
@@ -373,10 +378,10 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         if (w < we) {
             unsigned long long word;
             if (bottomUp) {
-                word = fout[w];
+                word = c0 == ws ? firstFout : fout[w];
             } else {
                 // the word's 8 threads are lanes of one wave: all load before its store
-                word = a.next[w];
+                word = c0 == ws ? firstNext : a.next[w];
                 if (b == 0) {
                     if (word) {
                         a.next[w] = 0ull;

@@ -20,16 +20,64 @@ struct DirectionParams {
 // Reads what level L-1 produced (per-segment counters of slot L % 3), decides level
 // L's direction and returns false when the previous level discovered nothing (then
 // block 0 records the final totals and raises ctrl->done).  Every thread of the block
-// must call it.
+// must call it.  When pre is given it also receives the inclusive prefix of the list
+// counts (*total = their sum), and cpre likewise of the chunk counts.
+//
+// Everything is read as one batch of independent loads (the done flag, the state of
+// level L-1, and each counter row once, thread t taking a contiguous run of segments),
+// then reduced and scanned in LDS: a kernel's preamble is one memory round trip, not a
+// chain of them (measured: 8.5 us of a small level's 9-13 us expand kernel went to the
+// direction sums and the two prefix loads done one after another).
 __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint32_t segs, unsigned L,
-                                            const DirectionParams& p, bool& bottomUp) {
+                                            const DirectionParams& p, bool& bottomUp, unsigned* pre = nullptr,
+                                            unsigned* total = nullptr, unsigned* cpre = nullptr,
+                                            unsigned* ctotal = nullptr) {
     __shared__ unsigned long long s_tmp[16];
-    if (ld_relaxed(&c->done)) return false;
+    __shared__ unsigned s_tmp32[16];
+    const unsigned t = threadIdx.x, nb = blockDim.x;
+    const unsigned per = (segs + nb - 1) / nb;
+    const size_t base = size_t(L % 3) * segs;
+    const unsigned doneIn = ld_relaxed(&c->done);
     const LevelState in = c->state[(L - 1) & 1];
-    const unsigned sIn = L % 3;
-    const unsigned long long nf = block_sum_row_u32(sc.news + size_t(sIn) * segs, segs, s_tmp);
-    const unsigned long long mf = block_sum_row_u64(sc.edges + size_t(sIn) * segs, segs, s_tmp);
-    __syncthreads();  // every thread has read state / counters before block 0 writes
+    unsigned long long nf = 0, mf = 0;
+    unsigned mine = 0, cmine = 0;  // this thread's running list / chunk counts
+    for (unsigned j0 = 0; j0 < per; j0 += 4) {
+        uint32_t nv[4], cv[4], kv[4];
+        unsigned long long ev[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t i = min(t * per + j0 + j, segs - 1);  // always in range: loads unconditional
+            nv[j] = sc.news[base + i];
+            ev[j] = sc.edges[base + i];
+            cv[j] = pre ? sc.count[base + i] : 0u;
+            kv[j] = cpre ? sc.chunks[base + i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t i = t * per + j0 + j;
+            if (j0 + j < per && i < segs) {
+                nf += nv[j];
+                mf += ev[j];
+                mine += cv[j];
+                cmine += kv[j];
+                if (pre) pre[i] = mine;
+                if (cpre) cpre[i] = cmine;
+            }
+        }
+    }
+    nf = block_sum_u64(nf, s_tmp);
+    mf = block_sum_u64(mf, s_tmp);
+    if (pre) {
+        const unsigned ex = block_exclusive_scan(mine, s_tmp32, *total);
+        for (unsigned j = 0; j < per && t * per + j < segs; ++j) pre[t * per + j] += ex;
+    }
+    if (cpre) {
+        const unsigned ex = block_exclusive_scan(cmine, s_tmp32, *ctotal);
+        for (unsigned j = 0; j < per && t * per + j < segs; ++j) cpre[t * per + j] += ex;
+    }
+    // one barrier: the prefixes are complete, the done flag is the block's, and every
+    // thread has read state / counters before block 0 writes
+    if (__syncthreads_or(doneIn != 0)) return false;
     if (nf == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             c->levels = L - 1;

@@ -205,7 +205,6 @@ __global__ void __launch_bounds__(kProducerBlock) __attribute__((amdgpu_waves_pe
 mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
-    __shared__ unsigned tmp[16];
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     const uint32_t segs = a.sv.segs;
@@ -217,15 +216,16 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     dp.totalEdges = (unsigned long long)a.k * g.nnz;
     dp.totalItems = double(g.n) * a.k;
     bool bottomUp = false;
-    if (!level_begin(c, a.sc, segs, L, dp, bottomUp)) return;
+    unsigned total = 0;  // list entries of the level (top-down)
+    if (!level_begin(c, a.sc, segs, L, dp, bottomUp, pre, &total)) return;
     const unsigned lane = lane_id();
     const unsigned long long* fin = mfront<NW>(a, g.n, L - 1);
     unsigned long long* fout = mfront<NW>(a, g.n, L);
     const bool writeLevels = L > a.histLevels;
-    const unsigned sIn = L % 3, sOut = (L + 1) % 3;
+    const unsigned sOut = (L + 1) % 3;
     if (!bottomUp) {
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
-        const unsigned total = load_prefix(a.sc.count + size_t(sIn) * segs, segs, pre, tmp);
+
         const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
         // m list entries per wave step (a short list is spread over as many waves)
