@@ -65,7 +65,7 @@ def reference_level_bytes(n, col_ptrs, rows, bv, levels, bu_mask, entry_row=None
       bottom-up B = 4 W + n/8 + sum_{u unvisited} (8 + 4 e_u) + 12 W + 4 |F'|
     Levels 1 .. max+1 (the last one finds nothing, as the traversal's last launch).
     entry_row (optional, reused across roots): the CSR row of every entry."""
-    lv = np.asarray(levels, dtype=np.int64)
+    lv = np.asarray(levels, dtype=np.int32)
     cp = np.asarray(col_ptrs, dtype=np.int64)
     W = (n + 31) // 32
     rp = bv["real_ptrs"].astype(np.int64)
@@ -76,7 +76,7 @@ def reference_level_bytes(n, col_ptrs, rows, bv, levels, bu_mask, entry_row=None
     top = int(lv.max()) if (lv >= 0).any() else 0
     reached = lv >= 0
     # per level: distinct rsets holding a vertex of that level
-    keys = np.unique(lv[reached] * R + (np.nonzero(reached)[0] >> 3))
+    keys = np.unique(lv[reached].astype(np.int64) * R + (np.nonzero(reached)[0] >> 3))
     kl, kr = keys // R, keys % R
     lvs_vsets = np.bincount(kl, weights=rvs[kr], minlength=top + 2)
     lvs_slices = np.bincount(kl, weights=rsl[kr], minlength=top + 2)
@@ -90,7 +90,7 @@ def reference_level_bytes(n, col_ptrs, rows, bv, levels, bu_mask, entry_row=None
             b = 5 * lvs_slices[L - 1] + 25 * lvs_vsets[L - 1] + 12 * W + 4 * newv + 4 * cnt2
         else:
             if entry_row is None:
-                entry_row = np.repeat(np.arange(n, dtype=np.int64), np.diff(cp))
+                entry_row = np.repeat(np.arange(n, dtype=np.int32), np.diff(cp))
             open_ = (lv < 0) | (lv >= L)
             unv = int(open_.sum())
             hits = np.flatnonzero(open_[entry_row] & (lv[rows] == L - 1))
@@ -513,7 +513,7 @@ def main():
         # HIP event around every kernel).
         t = time.time()
         replay_roots = roots if a.roofline_roots <= 0 else roots[:a.roofline_roots]
-        entry_row = np.repeat(np.arange(prep.csc.n, dtype=np.int64), np.diff(cp.astype(np.int64)))
+        entry_row = np.repeat(np.arange(prep.csc.n, dtype=np.int32), np.diff(cp.astype(np.int64)))
         tr = (B._lib.LevelTrace * 4096)()
         cnt = C.c_uint32()
         st = B._lib.BfsStats()

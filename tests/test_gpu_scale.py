@@ -94,3 +94,22 @@ def test_cc_parity_multiword(blest, oracle, name, algo):
     labels, count = oracle.cc(n, cp, rows)
     np.testing.assert_array_equal(res.components, labels, err_msg=name)
     assert res.no_components == count
+
+
+@pytest.mark.parametrize("name", ["rmat18_jaccard", "rmat18"])
+def test_bfs_hub_root_chunked_lists(blest, oracle, name):
+    """Roots in rsets of more than 16 vsets: the source's own rset is listed as chunk
+    entries by the init kernel, and its neighbours' hub rsets by the update kernel."""
+    say(f"case scale bfs hub roots {name}")
+    cp, rows, inv, b, g = prepared(blest, name)
+    n = len(cp) - 1
+    rp = b.arrays()["real_ptrs"].astype(np.int64)
+    vsets = np.diff(rp)
+    new_of_old = np.arange(n) if inv is None else np.asarray(inv)
+    deg = np.diff(cp.astype(np.int64))
+    hubs = [int(v) for v in np.argsort(deg)[::-1][:64] if vsets[new_of_old[v] >> 3] > 16][:4]
+    assert hubs, "the graph must have roots in rsets of more than 16 vsets"
+    k = blest.BVSSBFSKernel(blest.DeviceGraph(b, g, inv))
+    for src in hubs:
+        res = k.run(src)
+        np.testing.assert_array_equal(res.levels, oracle.bfs(n, cp, rows, src), err_msg=f"{name} hub src={src}")
