@@ -133,13 +133,22 @@ template <int NW>
 __device__ __forceinline__ unsigned segment_list(const GraphView& g, const SegView& sv,
                                                  const unsigned long long* f, uint32_t* list, ListLds& l) {
     const uint32_t ws = blockIdx.x * sv.wps, we = min(g.W, ws + sv.wps);
-    const unsigned wave = threadIdx.x / kWave, per = blockDim.x / 8 / (blockDim.x / kWave);
+    constexpr unsigned kPer = kWave / 8;  // words per wave per chunk (a chunk is blockDim / 8 words)
+    const unsigned wave = threadIdx.x / kWave;
     auto prep = [&](uint32_t c0) {
-        for (unsigned j = 0; j < per; ++j) {
-            const unsigned i = wave * per + j;
-            const uint32_t w = c0 + i;
-            const unsigned long long x = w < we ? activity<NW>(g, f, w) : 0ull;
-            if (lane_id() == 0) l.act[i] = x;
+        bool on[kPer];
+#pragma unroll
+        for (unsigned j = 0; j < kPer; ++j) {  // every word's loads first (at a valid address), then the ballots
+            const uint32_t w = c0 + wave * kPer + j, v = w * 64 + lane_id();
+            const bool ok = w < we && v < g.n;
+            Bits<NW> b;
+            b.load(f, ok ? v : 0u);
+            on[j] = ok && b.any();
+        }
+#pragma unroll
+        for (unsigned j = 0; j < kPer; ++j) {
+            const unsigned long long x = __ballot(on[j]);
+            if (lane_id() == 0) l.act[wave * kPer + j] = x;
         }
         __syncthreads();
     };
@@ -418,7 +427,7 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
 }
 
 template <int NW>
-__global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
+__global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_update_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ ListLds l;
     __shared__ unsigned long long tmp64[16];
@@ -441,37 +450,56 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_update_kernel(GraphView g
     wave_words(ws, we, threadIdx.x / kWave, blockDim.x / kWave, first, last);
     unsigned long long news = 0, edges = 0;
     const Bits<NW> rootMask = root_mask<NW>(a.k);
-    for (uint32_t w = first; w < last; ++w) {
-        const uint32_t v = w * 64 + lane_id();
-        Bits<NW> nw;
-        nw.zero();
-        bool nowFull = false;
-        if (v < g.n) {
-            nw.loadp(next_at<NW>(a, v));
-            if (nw.any()) {
-                Bits<NW> vis, z;
-                vis.loadp(vis_at<NW>(a, v));
-                z.zero();
-                z.storep(next_at<NW>(a, v));
-                nowFull = true;
+    // groups of kGroup words: the group's records (visited and next, one line per vertex)
+    // are all loaded before any is used, so a group costs one memory round trip
+    constexpr int kGroup = NW == 1 ? 4 : 2;  // (4 at 128 roots would cost the grid its residency)
+    for (uint32_t w0 = first; w0 < last; w0 += kGroup) {
+        Bits<NW> nx[kGroup], vs[kGroup];
+        bool ok[kGroup];
 #pragma unroll
-                for (int i = 0; i < NW; ++i) {
-                    vis.w[i] |= nw.w[i];
-                    nowFull &= vis.w[i] == rootMask.w[i];
-                }
-                vis.storep(vis_at<NW>(a, v));
-                const unsigned pc = nw.popc();
-                if (writeLevels)
-                    for (int i = 0; i < NW; ++i)
-                        for (unsigned long long b = nw.w[i]; b; b &= b - 1)
-                            a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
-                news += pc;
-                if (g.csrPtr) edges += (unsigned long long)(g.csrPtr[v + 1] - g.csrPtr[v]) * pc;
-            }
-            nw.store(fout, v);
+        for (int j = 0; j < kGroup; ++j) {
+            const uint32_t v = (w0 + j) * 64 + lane_id();
+            ok[j] = w0 + j < last && v < g.n;
+            const uint32_t vc = ok[j] ? v : 0u;
+            nx[j].loadp(next_at<NW>(a, vc));
+            vs[j].loadp(vis_at<NW>(a, vc));
         }
-        const unsigned long long fullBits = __ballot(nowFull);
-        if (fullBits && lane_id() == 0) a.full[w] |= fullBits;  // this wave owns word w here
+        uint32_t deg[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) {
+            const uint32_t v = (w0 + j) * 64 + lane_id();
+            deg[j] = (ok[j] && g.csrPtr && nx[j].any()) ? g.csrPtr[v + 1] - g.csrPtr[v] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) {
+            const uint32_t w = w0 + j, v = w * 64 + lane_id();
+            bool nowFull = false;
+            if (ok[j]) {
+                const Bits<NW>& nw = nx[j];
+                if (nw.any()) {
+                    Bits<NW> vis = vs[j], z;
+                    z.zero();
+                    z.storep(next_at<NW>(a, v));
+                    nowFull = true;
+#pragma unroll
+                    for (int i = 0; i < NW; ++i) {
+                        vis.w[i] |= nw.w[i];
+                        nowFull &= vis.w[i] == rootMask.w[i];
+                    }
+                    vis.storep(vis_at<NW>(a, v));
+                    const unsigned pc = nw.popc();
+                    if (writeLevels)
+                        for (int i = 0; i < NW; ++i)
+                            for (unsigned long long b = nw.w[i]; b; b &= b - 1)
+                                a.levels[size_t(64 * i + __builtin_ctzll(b)) * g.n + v] = int32_t(L);
+                    news += pc;
+                    edges += (unsigned long long)deg[j] * pc;
+                }
+                nw.store(fout, v);
+            }
+            const unsigned long long fullBits = __ballot(nowFull);
+            if (fullBits && lane_id() == 0 && w < last) a.full[w] |= fullBits;  // this wave owns word w here
+        }
     }
     __syncthreads();  // the frontier words above are read by other waves below
     const unsigned running = segment_list<NW>(g, a.sv, fout, lout, l);
