@@ -65,6 +65,8 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
             }
         }
     }
+    // levels enqueued past the end of the traversal stop here, before any reduction
+    if (__syncthreads_or(doneIn != 0)) return false;
     nf = block_sum_u64(nf, s_tmp);
     mf = block_sum_u64(mf, s_tmp);
     if (pre) {
@@ -75,9 +77,9 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
         const unsigned ex = block_exclusive_scan(cmine, s_tmp32, *ctotal);
         for (unsigned j = 0; j < per && t * per + j < segs; ++j) cpre[t * per + j] += ex;
     }
-    // one barrier: the prefixes are complete, the done flag is the block's, and every
-    // thread has read state / counters before block 0 writes
-    if (__syncthreads_or(doneIn != 0)) return false;
+    // the prefixes are complete and every thread has read state / counters before
+    // block 0 writes
+    __syncthreads();
     if (nf == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             c->levels = L - 1;
