@@ -38,7 +38,7 @@ struct BfsArgs {
     unsigned long long* visited;
     unsigned long long* front0;
     unsigned long long* front1;
-    unsigned long long* next;
+    uint8_t* nextB;  // top-down discoveries, one byte per vertex (W * 64 bytes)
     uint32_t* list0;
     uint32_t* list1;
     uint2* chunk0;  // chunk lists (rsets of more than kChunkVsets vsets)
@@ -65,14 +65,16 @@ __device__ __forceinline__ uint32_t front_byte(const unsigned long long* f, uint
     return reinterpret_cast<const uint8_t*>(f)[r];
 }
 
-// Slice (row, hit) of a frontier column: mark the row in `next` unless already seen.
-// (Issuing all of a lane's check loads before its atomics was measured slower here.)
+// Slice (row, hit) of a frontier column: a row not yet visited is marked with a plain
+// byte store (idempotent, so no atomic is needed).  A 64-bit atomicOr into a bitmap word
+// per hit serialised at the coherence point when a hub's dense rows met in the same
+// words: a hub root's first big top-down level took 46 us with the atomics and 17 us
+// without them.  The update kernel packs the bytes into bitmap words.
 __device__ __forceinline__ void expand_slice(uint32_t row, uint32_t hit, const unsigned long long* visited,
-                                             unsigned long long* next) {
+                                             uint8_t* nextB) {
     if (!hit) return;
-    const unsigned long long bit = 1ull << (row & 63);
-    if ((visited[row >> 6] | next[row >> 6]) & bit) return;
-    atomicOr(&next[row >> 6], bit);
+    if ((visited[row >> 6] >> (row & 63)) & 1ull) return;
+    nextB[row] = 1;
 }
 
 __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, BfsArgs a) {
@@ -83,8 +85,9 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
         const unsigned long long x = (w == sw) ? sb : 0ull;
         a.visited[w] = x;
         a.front1[w] = x;
-        a.next[w] = 0ull;
     }
+    unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
+    for (size_t i = tid; i < size_t(g.W) * 8; i += nthreads) nb64[i] = 0ull;
     const uint32_t srcSeg = sw / a.sv.wps;
     for (uint32_t s = tid; s < a.sv.segs; s += nthreads) {
         uint32_t cnt = 0, news = 0, chunks = 0;
@@ -175,10 +178,10 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                 }
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    expand_slice(r4[h].x, m[h] & 0xFF, a.visited, a.next);
-                    expand_slice(r4[h].y, (m[h] >> 8) & 0xFF, a.visited, a.next);
-                    expand_slice(r4[h].z, (m[h] >> 16) & 0xFF, a.visited, a.next);
-                    expand_slice(r4[h].w, (m[h] >> 24) & 0xFF, a.visited, a.next);
+                    expand_slice(r4[h].x, m[h] & 0xFF, a.visited, a.nextB);
+                    expand_slice(r4[h].y, (m[h] >> 8) & 0xFF, a.visited, a.nextB);
+                    expand_slice(r4[h].z, (m[h] >> 16) & 0xFF, a.visited, a.nextB);
+                    expand_slice(r4[h].w, (m[h] >> 24) & 0xFF, a.visited, a.nextB);
                 }
             }
         };
@@ -363,7 +366,8 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     const unsigned dir = c->state[L & 1].dir;
     const uint32_t listBase = a.sv.segBase[blockIdx.x], chunkBase = a.sv.chunkBase[blockIdx.x];
     const uint32_t w1 = min(ws + t / 8, g.W - 1);
-    const unsigned long long firstNext = a.next[w1], firstFout = fout[w1];
+    unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
+    const unsigned long long firstNB = nb64[size_t(w1) * 8 + b], firstFout = fout[w1];
     if (done) return;
     const bool bottomUp = dir != 0;
     uint32_t* out = ((L & 1) ? a.list0 : a.list1) + listBase;
@@ -375,21 +379,28 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         const uint32_t w = c0 + t / 8, r = w * 8 + b;
         uint32_t start = 0, cnt = 0, chunks = 0;
         bool any = false;
+        unsigned long long word = 0;
+        if (!bottomUp) {
+            // the word's discoveries: thread b holds bytes [64w + 8b, 64w + 8b + 8) of
+            // nextB, turns them into bits 8b..8b+7, and the word's 8 threads (aligned lanes
+            // of one wave) OR them together; each clears the bytes it read
+            const unsigned long long raw = w < we ? (c0 == ws ? firstNB : nb64[size_t(w) * 8 + b]) : 0ull;
+            unsigned m8 = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m8 |= ((raw >> (8 * i)) & 0xFFull) ? (1u << i) : 0u;
+            word = (unsigned long long)m8 << (8 * b);
+            word |= __shfl_xor(word, 1, kWave);
+            word |= __shfl_xor(word, 2, kWave);
+            word |= __shfl_xor(word, 4, kWave);
+            if (raw) nb64[size_t(w) * 8 + b] = 0ull;
+        }
         if (w < we) {
-            unsigned long long word;
             if (bottomUp) {
                 word = c0 == ws ? firstFout : fout[w];
-            } else {
-                // the word's 8 threads are lanes of one wave: all load before its store
-                word = c0 == ws ? firstNext : a.next[w];
-                if (b == 0) {
-                    if (word) {
-                        a.next[w] = 0ull;
-                        a.visited[w] |= word;
-                    }
-                    fout[w] = word;
-                    sWord[t / 8] = word;
-                }
+            } else if (b == 0) {
+                if (word) a.visited[w] |= word;
+                fout[w] = word;
+                sWord[t / 8] = word;
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
             any = byte != 0;
@@ -476,7 +487,7 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     a.visited = q.visited;
     a.front0 = q.front0;
     a.front1 = q.front1;
-    a.next = q.next;
+    a.nextB = q.nextB;
     a.list0 = q.list0;
     a.list1 = q.list1;
     a.chunk0 = q.chunk0;

@@ -337,6 +337,7 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
             q.front0 = q.alloc<unsigned long long>(g.W);
             q.front1 = q.alloc<unsigned long long>(g.W);
             q.next = q.alloc<unsigned long long>(g.W);
+            q.nextB = q.alloc<uint8_t>(size_t(g.W) * 64);
             q.list0 = q.alloc<uint32_t>(g.V);
             q.list1 = q.alloc<uint32_t>(g.V);
             q.chunk0 = q.alloc<uint2>(g.chunks);
@@ -391,7 +392,7 @@ blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_so
         uint64_t b = sizeof(dev::Ctrl);
         const uint64_t levels = 4ull * g.n * max_sources;
         b += levels * (g.hasPerm ? 2 : 1);
-        b += 4ull * 8ull * g.W + 2ull * 4ull * g.V + 2ull * 8ull * g.chunks + 3ull * 20ull * g.segs;
+        b += 4ull * 8ull * g.W + 64ull * g.W + 2ull * 4ull * g.V + 2ull * 8ull * g.chunks + 3ull * 20ull * g.segs;
         if (max_sources > 1) {
             const uint32_t nw = max_sources > 64 ? 2 : 1;
             b += (4ull + mbfs_history_levels(g.n, nw) + 1) * 8ull * g.n * nw + 4ull * 128 + 16ull * 128;

@@ -118,6 +118,7 @@ struct Query {
     dev::Ctrl* ctrl = nullptr;
     int32_t* levels = nullptr;  // n * maxSources
     unsigned long long *visited = nullptr, *front0 = nullptr, *front1 = nullptr, *next = nullptr;
+    uint8_t* nextB = nullptr;  // single-source top-down discoveries, a byte per vertex (W * 64)
     uint32_t *list0 = nullptr, *list1 = nullptr;
     uint2 *chunk0 = nullptr, *chunk1 = nullptr;        // chunk lists (single-source BFS)
     uint32_t *segCount = nullptr, *segNew = nullptr;  // [3][segs]
