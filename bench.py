@@ -8,7 +8,10 @@ A step is one BFS from every root of the workload (the graph already resident in
 Per-root GTEPS = m_c / t, with m_c = half the degree sum of the reached vertices
 (undirected edges of the root's component) and t the bfs_kernel launch measured with
 HIP events on the query's stream; the reported value is the harmonic mean over all
-(step, root) launches.  The line also carries:
+(step, root) launches.  Before timing, the native autotuner (blest_autotune_bfs) picks the
+launch shape and direction thresholds on --autotune-roots of the roots, as a deployment
+tunes once per resident graph (--no-autotune keeps the device defaults).  The line also
+carries:
   roofline      algorithmic bytes per launch (the reference formulation's B_td / B_bu,
                 computed by reference_level_bytes from a sample of launches' own GPU
                 levels and direction schedules) / those launches' mean time, against
@@ -397,7 +400,10 @@ def main():
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
-    ap.add_argument("--autotune", action="store_true", help="pick blocks_per_cu / alpha / beta with the autotuner")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick blocks_per_cu / alpha / beta with the native autotuner on --autotune-roots "
+                         "of the workload's roots before timing (default on; --no-autotune keeps the "
+                         "device defaults)")
     ap.add_argument("--autotune-roots", type=int, default=32)
     ap.add_argument("--use-graphs", type=int, default=-1,
                     help="replay captured level chunks: 0 after a deep traversal, 1 always, 2 never (default)")
@@ -455,7 +461,7 @@ def main():
         print(json.dumps(mbfs_bench(a, prep, opts)), flush=True)
         return
     tuned = None
-    if a.autotune:
+    if a.autotune and not (a.blocks_per_cu or a.alpha or a.beta):
         # launch shape and switch thresholds chosen by the native autotuner on some of the roots
         t0 = time.time()
         best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:a.autotune_roots],
