@@ -109,16 +109,6 @@ __device__ __forceinline__ unsigned long long* mfront(const MbfsArgs& a, uint32_
     return (L & 1) ? a.front1 : a.front0;
 }
 
-// Activity word of vertex word w: bit l = some root has vertex 64w+l in frontier f.
-template <int NW>
-__device__ __forceinline__ unsigned long long activity(const GraphView& g, const unsigned long long* f, uint32_t w) {
-    const uint32_t v = w * 64 + lane_id();
-    Bits<NW> b;
-    b.zero();
-    if (v < g.n) b.load(f, v);
-    return __ballot(b.any());
-}
-
 // The vset list of a frontier held as per-vertex words, for this workgroup's segment,
 // written by the whole workgroup (block_write_vsets); each chunk's 64 word activities
 // are first staged in LDS (wave w computes words 8w .. 8w+7).  Returns the count.

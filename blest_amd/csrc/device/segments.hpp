@@ -83,20 +83,6 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x
     return s;
 }
 
-// Sum of a [segs] counter row, computed by the whole block (every thread gets it).
-__device__ __forceinline__ unsigned long long block_sum_row_u32(const uint32_t* row, uint32_t segs,
-                                                                unsigned long long* tmp) {
-    unsigned long long s = 0;
-    for (uint32_t i = threadIdx.x; i < segs; i += blockDim.x) s += row[i];
-    return block_sum_u64(s, tmp);
-}
-__device__ __forceinline__ unsigned long long block_sum_row_u64(const unsigned long long* row, uint32_t segs,
-                                                                unsigned long long* tmp) {
-    unsigned long long s = 0;
-    for (uint32_t i = threadIdx.x; i < segs; i += blockDim.x) s += row[i];
-    return block_sum_u64(s, tmp);
-}
-
 // Loads a [segs] count row into LDS as an inclusive prefix (pre[i] = sum of counts
 // 0..i); returns the total.  pre holds kMaxSegs entries.
 __device__ __forceinline__ unsigned load_prefix(const uint32_t* row, uint32_t segs, unsigned* pre, unsigned* tmp) {
@@ -149,14 +135,6 @@ __device__ __forceinline__ unsigned word_vsets(const uint32_t* rsetUnit, uint32_
         for (int b = 0; b < 8; ++b)
             if ((bits >> (8 * b)) & 0xFFull) cnt += rsetUnit[w * 8 + b + 1] - rsetUnit[w * 8 + b];
     return cnt;
-}
-
-__device__ __forceinline__ void write_word_vsets(const uint32_t* rsetUnit, uint32_t w, unsigned long long bits,
-                                                 uint32_t* out) {
-    if (!bits) return;
-    for (int b = 0; b < 8; ++b)
-        if ((bits >> (8 * b)) & 0xFFull)
-            for (uint32_t u = rsetUnit[w * 8 + b]; u < rsetUnit[w * 8 + b + 1]; ++u) *out++ = u;
 }
 
 // Writes, at out, the vsets of the rsets active in words [first, last) (in word and
@@ -281,36 +259,6 @@ __device__ __forceinline__ void wave_words(uint32_t ws, uint32_t we, unsigned wi
     const uint32_t per = (we - ws + wpb - 1) / wpb;
     first = min(we, ws + wib * per);
     last = min(we, first + per);
-}
-
-// Wave-level (lane = one of the word's 8 rsets) count of the vsets of the rsets with a
-// frontier vertex in `bits` (every lane gets the total).
-__device__ __forceinline__ unsigned wave_word_vsets(const uint32_t* rsetUnit, uint32_t w, unsigned long long bits) {
-    const unsigned l = lane_id();
-    unsigned mine = 0;
-    if (l < 8 && ((bits >> (8 * l)) & 0xFFull)) mine = rsetUnit[w * 8 + l + 1] - rsetUnit[w * 8 + l];
-    return wave_sum_u32(mine);
-}
-
-// Writes those vsets at out[0 ..), lanes 0..7 one rset each; returns the count.
-__device__ __forceinline__ unsigned wave_write_word_vsets(const uint32_t* rsetUnit, uint32_t w,
-                                                          unsigned long long bits, uint32_t* out) {
-    const unsigned l = lane_id();
-    unsigned mine = 0, start = 0;
-    if (l < 8 && ((bits >> (8 * l)) & 0xFFull)) {
-        start = rsetUnit[w * 8 + l];
-        mine = rsetUnit[w * 8 + l + 1] - start;
-    }
-    const unsigned incl = wave_inclusive_scan(mine);
-    for (unsigned u = 0; u < mine; ++u) out[incl - mine + u] = start + u;
-    return __shfl(incl, kWave - 1, kWave);
-}
-
-// Offset of this wave's output inside the segment, from each wave's count (one block
-// scan per kernel).  Returns the segment total in `total`.
-__device__ __forceinline__ unsigned wave_offset_in_block(unsigned waveCount, unsigned* tmp, unsigned& total) {
-    const unsigned ex = block_exclusive_scan(lane_id() == 0 ? waveCount : 0u, tmp, total);
-    return __shfl(ex, 0, kWave);
 }
 
 }  // namespace dev
