@@ -10,18 +10,21 @@
 // per producer workgroup (segments.hpp) so that no level funnels through one atomic.
 //
 // Level L (frontier F = vertices at distance L-1):
-//   expand  every block derives the same direction from the previous level's counts;
-//           top-down: each wave takes 64 frontier vsets, scans their quad counts
-//             across the wave and streams the union of their quads 64 at a time
-//             (every lane busy, loads coalesced); a slice hits when
-//             (mask & frontier byte) != 0; rows not yet visited are OR-ed into
-//             `next` (64-bit atomics, skipped when the bit is already there);
-//           bottom-up: workgroup per segment, wave per 64-vertex word, lane per
-//             vertex: an unvisited vertex scans its in-neighbours until one is in
-//             the frontier; the wave's ballot is the new frontier word.
-//   update  workgroup per segment, thread per word: top-down folds `next` into
-//           visited / frontier / levels; both directions write the vsets of the
-//           rsets with new vertices into the segment's list range.
+//   expand  every block derives the same direction from the previous level's counts
+//           (one batch of loads, level_common.hpp);
+//           top-down: each wave step takes m list entries (vsets) or chunk slots (a
+//             hub rset's vsets, 16 to a chunk), scans their quad counts across the
+//             wave and streams the union of their quads (every lane busy, loads
+//             coalesced); a slice hits when (mask & frontier byte) != 0, and a hit row
+//             not yet visited gets a plain byte store in `nextB` (no atomics);
+//           bottom-up: workgroup per segment; a lane owns one vertex in each of 4 words
+//             of its wave and scans their in-neighbours one vertex after another until
+//             one is in the frontier (1, 2, then 4 entries per step); a scan open after
+//             32 entries is finished 64 entries at a time by a whole wave.
+//   update  workgroup per segment, thread per rset: top-down packs each word's 64
+//           discovery bytes into the new frontier word and folds it into visited /
+//           levels; both directions list the vsets of the rsets with new vertices in
+//           the segment's list range (hub rsets as chunks in its chunk range).
 // Direction: top-down -> bottom-up when the frontier's edges exceed the unexplored
 // edges / alpha; back when the frontier shrinks below n / beta.  Levels are unique
 // BFS distances, so every schedule produces the same result.
