@@ -406,7 +406,7 @@ def main():
                          "device defaults)")
     ap.add_argument("--autotune-roots", type=int, default=32)
     ap.add_argument("--use-graphs", type=int, default=-1,
-                    help="replay captured level chunks: 0 after a deep traversal, 1 always, 2 never (default)")
+                    help="replay captured level chunks: 0 after a deep traversal (default), 1 always, 2 never")
     ap.add_argument("--serve-scales", default="", help="serve: comma-separated R-MAT scales (default 18..25)")
     ap.add_argument("--serve-grids", default="", help="serve: comma-separated grid sides (default 2048,4096)")
     ap.add_argument("--serve-requests", type=int, default=2000)

@@ -133,7 +133,7 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->direction = 0;
     o->keep_csr = 1;
     o->timeout_ms = 20000;
-    o->use_graphs = 2;
+    o->use_graphs = 0;
 }
 
 blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,

@@ -181,8 +181,8 @@ typedef struct blest_device_opts {
     int keep_csr;            /* upload the bottom-up CSR (needed for direction != 1) */
     uint32_t timeout_ms;     /* bound on every in-kernel wait */
     int use_graphs;          /* level chunks replayed from a captured HIP graph: 0 = once a
-                                query's previous traversal was >= 64 levels deep, 1 = always,
-                                2 = never (default).  Queries of a request server never replay. */
+                                query's previous traversal was >= 64 levels deep (default),
+                                1 = always, 2 = never.  Queries of a request server never replay. */
 } blest_device_opts;
 void blest_device_opts_default(blest_device_opts* o);
 

@@ -16,7 +16,7 @@ def lib(blest):
 def test_device_opts_defaults(blest):
     o = blest.device_opts()
     assert (o.blocks_per_cu, o.alpha, o.beta, o.direction, o.keep_csr) == (4, 15.0, 18.0, 0, 1)
-    assert o.use_graphs == 2  # captured level chunks only on request
+    assert o.use_graphs == 0  # captured level chunks replayed once a traversal was deep
 
 
 def test_server_opts_defaults(blest, lib):
