@@ -85,6 +85,11 @@ static bool poll_done(Query& q, const std::function<void()>& tail) {
     return *q.hostDone != 0;
 }
 
+// The first chunk covers the previous traversal's depth (levelHint) plus two levels.  A
+// traversal that goes deeper (another root) is usually only a little deeper, so the
+// chunks after the first start small and double: a miss costs one more poll, not a
+// doubled chunk of levels that only find the done flag (on the 4096-square grid, with
+// depths of 60-80 levels, doubling launched up to 230 levels for a 75-level traversal).
 void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail,
                 LevelGraph* lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
     const unsigned limit = q.g->n + 2;
@@ -92,23 +97,26 @@ void run_levels(Query& q, const std::function<void(unsigned)>& level, const std:
     if (lg && levelRel && !q.noGraphs && (mode == 1 || (mode == 0 && q.levelHint >= kGraphMinLevels))) {
         if (!lg->exec || lg->key != key) capture_levels(q, *lg, levelRel, key);
         const unsigned C = lg->chunk;
+        const unsigned maxReps = std::max(1u, 4096u / C);
         unsigned launched = 0;
-        unsigned reps = std::max(1u, (q.levelHint + 2 + C - 1) / C);
+        unsigned reps = std::max(1u, (q.levelHint + 2 + C - 1) / C), more = 1;
         for (;;) {
             for (unsigned i = 0; i < reps && launched <= limit; ++i, launched += C)
                 BLEST_HIP(hipGraphLaunch(lg->exec, q.stream));
             if (poll_done(q, tail)) return;
             if (launched > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
-            reps = std::min(reps * 2, std::max(1u, 4096u / C));
+            reps = more;
+            more = std::min(more * 2, maxReps);
         }
     }
     unsigned L = 1;
-    unsigned chunk = std::max(4u, q.levelHint + 2);
+    unsigned chunk = std::max(4u, q.levelHint + 2), more = std::max(4u, q.levelHint / 8);
     for (;;) {
         for (unsigned i = 0; i < chunk && L <= limit; ++i) level(L++);
         if (poll_done(q, tail)) break;
         if (L > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
-        chunk = std::min(chunk * 2, 4096u);
+        chunk = more;
+        more = std::min(more * 2, 4096u);
     }
 }
 

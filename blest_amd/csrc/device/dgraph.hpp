@@ -104,7 +104,7 @@ struct LevelGraph {
     uint64_t key = 0;
 };
 enum LevelGraphKind { kGraphBfs = 0, kGraphMbfs1 = 1, kGraphMbfs2 = 2, kGraphCount = 3 };
-constexpr unsigned kGraphChunkLevels = 32;  // levels per replayed graph
+constexpr unsigned kGraphChunkLevels = 16;  // levels per replayed graph
 constexpr unsigned kGraphMinLevels = 64;    // auto mode: replay once a traversal was this deep
 
 // Mutable per-query device state.
