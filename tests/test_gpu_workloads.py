@@ -6,6 +6,7 @@ bit-exactly with the oracle on the original graph: BFS levels in original IDs (-
 unreachable) for the benchmark's own roots, canonical CC labels, and multi-source rows.
 R-MAT 22 under Gorder+Jaccard needs its cached ordering (computing it takes far longer
 than a test may); without the cache those cases are skipped."""
+import os
 import sys
 
 import numpy as np
@@ -84,3 +85,30 @@ def test_mbfs_128_roots_rmat22(blest, oracle):
     for j in (0, 31, 63, 64, 100, 127):
         np.testing.assert_array_equal(res.level_arrays[j], oracle.bfs(n, p.orig_col_ptrs, p.orig_rows, int(roots[j])),
                                       err_msg=f"row {j} root {roots[j]}: multi-source vs oracle")
+
+
+@pytest.mark.skipif(os.environ.get("BLEST_LARGE_TESTS") != "1",
+                    reason="R-MAT 26 (67 M vertices, 2.1 B edges) takes minutes; set BLEST_LARGE_TESTS=1")
+def test_rmat26_bfs_mbfs_cc(blest, oracle):
+    """R-MAT 26, natural order, as benchmarked: BFS from 2 of the benchmark's roots and CC
+    against the oracle, and a 64-root multi-source launch (holding those 2 roots) whose
+    every row equals the single-source GPU BFS of its root."""
+    say("case workload rmat26 (large)")
+    p = workload("rmat26")
+    n = p.csc.n
+    cp, rows = p.orig_col_ptrs, p.orig_rows
+    single = blest.BVSSBFSKernel(device_graph(blest, p))
+    roots = [int(r) for r in p.roots[:64]]
+    for src in roots[:2]:
+        say(f"  bfs root {src}")
+        np.testing.assert_array_equal(single.run(src).levels, oracle.bfs(n, cp, rows, src), err_msg=f"src={src}")
+    say("  cc")
+    res = blest.BVSSCCKernel(p.bvss, p.csc, p.inv_perm).run()
+    labels, count = oracle.cc(n, cp, rows)
+    np.testing.assert_array_equal(res.components, labels)
+    assert res.no_components == count
+    del labels, res
+    say("  mbfs 64 roots")
+    m = blest.BVSSMBFSKernel(p.bvss, p.csc, p.inv_perm, batch=64).run(np.asarray(roots, dtype=np.uint32))
+    for j, r in enumerate(roots):
+        np.testing.assert_array_equal(m.level_arrays[j], single.run(r).levels, err_msg=f"row {j} root {r}")
