@@ -91,7 +91,7 @@ def test_mbfs_128_roots_rmat22(blest, oracle):
                     reason="R-MAT 26 (67 M vertices, 2.1 B edges) takes minutes; set BLEST_LARGE_TESTS=1")
 def test_rmat26_bfs_mbfs_cc(blest, oracle):
     """R-MAT 26, natural order, as benchmarked: BFS from 2 of the benchmark's roots and CC
-    against the oracle, and a 64-root multi-source launch (holding those 2 roots) whose
+    against the oracle, and a 128-root multi-source launch (holding those 2 roots) whose
     every row equals the single-source GPU BFS of its root."""
     say("case workload rmat26 (large)")
     p = workload("rmat26")
@@ -108,7 +108,11 @@ def test_rmat26_bfs_mbfs_cc(blest, oracle):
     np.testing.assert_array_equal(res.components, labels)
     assert res.no_components == count
     del labels, res
-    say("  mbfs 64 roots")
-    m = blest.BVSSMBFSKernel(p.bvss, p.csc, p.inv_perm, batch=64).run(np.asarray(roots, dtype=np.uint32))
-    for j, r in enumerate(roots):
+    # 128 roots, as benchmarked: the benchmark's 64 and 64 more (two words per vertex)
+    extra = [int(r) for r in blest.CSC.from_arrays(n, cp, rows).pick_roots(192, seed=5) if int(r) not in set(roots)]
+    batch = roots + extra[:128 - len(roots)]
+    say(f"  mbfs {len(batch)} roots")
+    m = blest.BVSSMBFSKernel(p.bvss, p.csc, p.inv_perm, batch=128).run(np.asarray(batch, dtype=np.uint32))
+    assert len(m.level_arrays) == 128
+    for j, r in enumerate(batch):
         np.testing.assert_array_equal(m.level_arrays[j], single.run(r).levels, err_msg=f"row {j} root {r}")
