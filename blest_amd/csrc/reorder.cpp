@@ -191,7 +191,13 @@ std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uin
                     S.rowStart[r] = uint32_t(std::lower_bound(b, e, ws) - t->rows.data());
                     S.rowWin[r] = uint32_t(w) + 1;
                 }
-                for (uint32_t q = S.rowStart[r]; q < t->colPtrs[r + 1]; ++q) {
+                // placed columns stay placed for the rest of the window: a leading run of
+                // them is skipped for good by advancing the row's start
+                uint32_t q = S.rowStart[r];
+                const uint32_t qe = t->colPtrs[r + 1];
+                while (q < qe && t->rows[q] < we && S.placed[t->rows[q] - ws]) ++q;
+                S.rowStart[r] = q;
+                for (; q < qe; ++q) {
                     uint32_t c = t->rows[q];
                     if (c >= we) break;
                     if (S.placed[c - ws]) continue;
