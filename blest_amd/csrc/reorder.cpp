@@ -148,8 +148,14 @@ std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uin
         std::vector<uint32_t> rowStart;  // per row: first transpose position >= windowStart
         std::vector<uint32_t> rowWin;    // per row: window for which rowStart is valid (+1)
         std::vector<uint32_t> bySize;    // window columns by (degree desc, id desc)
+        std::vector<uint32_t> rowLen;    // per row: kPooled | length of its run in `pool`
+        std::vector<uint32_t> pool;      // long rows' window columns, unplaced ones first
         uint32_t epoch = 0;
     };
+    // Rows with at least kLongRun columns in a window get a private copy of that run,
+    // compacted whenever more than half of it has been placed (ascending order kept, so
+    // the candidate queue fills in the same order).
+    constexpr uint32_t kLongRun = 64, kPooled = 0x80000000u;
     std::vector<Scratch> scratch(threads);
     parallel_dynamic(noWindows, threads, 1, [&](uint64_t w, unsigned tid) {
         Scratch& S = scratch[tid];
@@ -159,7 +165,9 @@ std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uin
             S.rowMark.assign(n, 0);
             S.rowStart.assign(n, 0);
             S.rowWin.assign(n, 0);
+            S.rowLen.assign(n, 0);
         }
+        S.pool.clear();
         S.placed.assign(len, 0);
         S.inter.assign(len, 0);
         S.queue.clear();
@@ -188,8 +196,36 @@ std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uin
                 if (S.rowWin[r] != w + 1) {
                     const uint32_t* b = t->rows.data() + t->colPtrs[r];
                     const uint32_t* e = t->rows.data() + t->colPtrs[r + 1];
-                    S.rowStart[r] = uint32_t(std::lower_bound(b, e, ws) - t->rows.data());
+                    const uint32_t* lo = std::lower_bound(b, e, ws);
+                    const uint32_t* hi = std::lower_bound(lo, e, we);
+                    S.rowStart[r] = uint32_t(lo - t->rows.data());
+                    S.rowLen[r] = 0;
+                    if (uint32_t(hi - lo) >= kLongRun) {
+                        S.rowStart[r] = uint32_t(S.pool.size());
+                        S.rowLen[r] = kPooled | uint32_t(hi - lo);
+                        S.pool.insert(S.pool.end(), lo, hi);
+                    }
                     S.rowWin[r] = uint32_t(w) + 1;
+                }
+                if (S.rowLen[r] & kPooled) {
+                    uint32_t* run = S.pool.data() + S.rowStart[r];
+                    const uint32_t runLen = S.rowLen[r] & ~kPooled;
+                    uint32_t seen = 0;  // placed columns met
+                    for (uint32_t i = 0; i < runLen; ++i) {
+                        const uint32_t c = run[i];
+                        if (S.placed[c - ws]) {
+                            ++seen;
+                            continue;
+                        }
+                        if (S.inter[c - ws]++ == 0) S.queue.push_back(c);
+                    }
+                    if (2 * seen > runLen) {
+                        uint32_t j = 0;
+                        for (uint32_t i = 0; i < runLen; ++i)
+                            if (!S.placed[run[i] - ws]) run[j++] = run[i];
+                        S.rowLen[r] = kPooled | j;
+                    }
+                    continue;
                 }
                 // placed columns stay placed for the rest of the window: a leading run of
                 // them is skipped for good by advancing the row's start
