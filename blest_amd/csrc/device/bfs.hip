@@ -603,20 +603,33 @@ blest_status blest_bfs(blest_query* qh, uint32_t src, int32_t* levels, blest_bfs
     });
 }
 
+// The traversal runs on the query's own stream.  A caller stream, when given, is joined
+// both ways: the traversal starts after the work already queued on it, and work queued
+// on it afterwards (e.g. a read of blest_query_levels_device) starts after the traversal.
 blest_status blest_bfs_async(blest_query* qh, uint32_t src, void* stream) {
     return guarded([&] {
         if (!qh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "query is null");
         Query& q = qh->q;
-        (void)stream;  // the query's own stream orders its launches
+        std::lock_guard<std::mutex> lk(q.mu);
         BLEST_HIP(hipSetDevice(q.g->device));
-        bfs_launch(q, map_source(*q.g, src));
+        const uint32_t s = map_source(*q.g, src);
+        hipStream_t caller = static_cast<hipStream_t>(stream);
+        if (caller) {
+            BLEST_HIP(hipEventRecord(q.evIn, caller));
+            BLEST_HIP(hipStreamWaitEvent(q.stream, q.evIn, 0));
+        }
+        bfs_launch(q, s);
+        if (caller) BLEST_HIP(hipStreamWaitEvent(caller, q.ev1, 0));
     });
 }
 
 blest_status blest_query_sync(blest_query* qh, blest_bfs_stats* st) {
     return guarded([&] {
         if (!qh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "query is null");
-        bfs_finish(qh->q, nullptr, st);
+        Query& q = qh->q;
+        std::lock_guard<std::mutex> lk(q.mu);
+        BLEST_HIP(hipSetDevice(q.g->device));
+        bfs_finish(q, nullptr, st);
     });
 }
 

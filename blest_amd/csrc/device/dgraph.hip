@@ -22,6 +22,7 @@ Query::~Query() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (evChunk) (void)hipEventDestroy(evChunk);
+    if (evIn) (void)hipEventDestroy(evIn);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -337,6 +338,7 @@ blest_status blest_query_create(blest_device_graph* dg, uint32_t max_sources, bl
             BLEST_HIP(hipEventCreate(&q.ev0));
             BLEST_HIP(hipEventCreate(&q.ev1));
             BLEST_HIP(hipEventCreateWithFlags(&q.evChunk, hipEventDisableTiming));
+            BLEST_HIP(hipEventCreateWithFlags(&q.evIn, hipEventDisableTiming));
             BLEST_HIP(hipHostMalloc(reinterpret_cast<void**>(&q.hostDone), sizeof(unsigned), hipHostMallocDefault));
             q.ctrl = q.alloc<dev::Ctrl>(1);
             BLEST_HIP(hipMemset(q.ctrl, 0, sizeof(dev::Ctrl)));

@@ -113,6 +113,7 @@ struct Query {
     uint32_t maxSources = 1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evChunk = nullptr;
+    hipEvent_t evIn = nullptr;     // a caller's stream position that the next launch waits for
     unsigned* hostDone = nullptr;  // pinned: the traversal's done flag, polled per chunk
     unsigned levelHint = 0;        // levels of the last traversal (sizes the first chunk)
     dev::Ctrl* ctrl = nullptr;

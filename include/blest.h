@@ -237,7 +237,11 @@ typedef struct blest_bfs_stats {
  * the graph's device; the copy is ordered on the query's stream and complete on return. */
 blest_status blest_bfs(blest_query* q, uint32_t src, int32_t* levels, blest_bfs_stats* st);
 /* Same traversal without the device-to-host copy: the levels stay in the query's
- * device buffer (blest_query_levels_device). stream: hipStream_t or NULL. */
+ * device buffer (blest_query_levels_device).  stream: hipStream_t or NULL.  The
+ * traversal runs on the query's own stream; a non-null caller stream is joined both
+ * ways (the traversal starts after the work already queued on it, and work queued on
+ * it afterwards starts after the traversal).  blest_query_sync waits for the
+ * traversal and fills st.  Both take the query's lock like every other call on it. */
 blest_status blest_bfs_async(blest_query* q, uint32_t src, void* stream);
 blest_status blest_query_sync(blest_query* q, blest_bfs_stats* st);
 /* Device pointer (int32*) of the query's level array (n * max_sources entries). */
