@@ -69,8 +69,9 @@ def symmetric_csc(n: int, u: np.ndarray, v: np.ndarray) -> tuple[np.ndarray, np.
     return np.cumsum(cp).astype(np.uint32), rws.astype(np.uint32)
 
 
-def rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, permute: bool):
-    n = 1 << scale
+def rmat_samples(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int):
+    """The raw R-MAT edge samples (u, v) before relabelling and symmetrisation, and the
+    per-level quadrant codes (bit 1 = u's bit, bit 0 = v's bit; level 0 = top bit)."""
     m = edge_factor << scale
     kk = key(seed, 1)
     pairs = (scale + 1) // 2
@@ -79,6 +80,7 @@ def rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, 
     sv = np.zeros(m, dtype=np.int64)
     ab, abc = a + b, a + b + c
     h = None
+    quads = []
     for lvl in range(scale):
         if lvl % 2 == 0:
             h = draw(kk, ks * np.uint64(pairs) + np.uint64(lvl // 2))
@@ -90,6 +92,13 @@ def rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, 
         bv = (((r >= a) & (r < ab)) | (r >= abc)).astype(np.int64)  # quadrants b, d
         su = (su << 1) | bu
         sv = (sv << 1) | bv
+        quads.append((bu << 1 | bv).astype(np.uint8))
+    return su, sv, quads
+
+
+def rmat(scale: int, edge_factor: int, a: float, b: float, c: float, seed: int, permute: bool):
+    n = 1 << scale
+    su, sv, _ = rmat_samples(scale, edge_factor, a, b, c, seed)
     if permute:
         p = permutation(n, seed, 2).astype(np.int64)
         su, sv = p[su], p[sv]

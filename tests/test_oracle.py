@@ -283,3 +283,71 @@ def test_bench_byte_model_matches_oracle_replay(oracle, blest):
                 sched = [(mask >> i) & 1 for i in range(64)]
                 lv, _, per = oracle.bvss_bfs(g.n, bv, cp, rows, int(src), schedule=sched, threads=2)
                 assert bench.reference_level_bytes(g.n, cp, rows, bv, lv, mask) == [p["bytes"] for p in per]
+
+
+# ------------------------------------------------------------- generator distributions
+# The generator restatement is bit-exact with libblest's generators (test_host.py); these
+# checks pin the restatement itself against the distributions the workloads specify.
+def _binomial_ok(count, m, p, sigmas=6.0):
+    return abs(count - m * p) <= sigmas * np.sqrt(m * p * (1 - p)) + 1
+
+
+@pytest.mark.parametrize("a,b,c", [(0.57, 0.19, 0.19), (0.25, 0.25, 0.25), (0.45, 0.15, 0.3)])
+def test_rmat_quadrant_frequencies(a, b, c):
+    """Every recursion level draws quadrant a/b/c/d with the configured probabilities, and
+    consecutive levels (the two halves of one hash) are independent."""
+    from oracle import gen
+    scale, ef = 12, 16
+    _, _, quads = gen.rmat_samples(scale, ef, a, b, c, seed=11)
+    m = ef << scale
+    probs = [a, b, c, 1 - a - b - c]
+    for lvl, q in enumerate(quads):
+        counts = np.bincount(q, minlength=4)
+        for k in range(4):
+            assert _binomial_ok(counts[k], m, probs[k]), (lvl, k, counts[k], m * probs[k])
+    for lvl in range(scale - 1):
+        joint = np.bincount(quads[lvl].astype(np.int64) * 4 + quads[lvl + 1], minlength=16)
+        for i in range(4):
+            for j in range(4):
+                assert _binomial_ok(joint[4 * i + j], m, probs[i] * probs[j]), (lvl, i, j)
+
+
+def test_rmat_degree_skew_matches_quadrant_mass():
+    """Closed form: the expected share of raw samples whose source lies in the lower half
+    of the ID range is a + b at every level; with a = 0.57, b = 0.19 that is 0.76."""
+    from oracle import gen
+    su, sv, _ = gen.rmat_samples(14, 16, 0.57, 0.19, 0.19, seed=3)
+    m = su.size
+    low_u = int((su < (1 << 13)).sum())
+    low_v = int((sv < (1 << 13)).sum())
+    assert _binomial_ok(low_u, m, 0.76) and _binomial_ok(low_v, m, 0.76)
+    # vertex 0 is the most likely endpoint: P(u = 0) = (a + b)^scale
+    assert (su == 0).sum() >= 0.5 * m * 0.76 ** 14
+
+
+def test_grid_edge_keep_rate_and_shortcuts():
+    from oracle import gen
+    side, keep = 300, 0.9
+    n, cp, rows = gen.grid(side, keep, 0.0, seed=5)
+    deg = np.diff(cp.astype(np.int64))
+    lattice = 2 * side * (side - 1)  # undirected grid edges
+    kept = int(deg.sum()) // 2
+    assert _binomial_ok(kept, lattice, keep)
+    n2, cp2, _ = gen.grid(side, keep, 0.01, seed=5)
+    extra = int(np.diff(cp2.astype(np.int64)).sum()) // 2 - kept
+    cnt = round(0.01 * n)
+    # shortcuts: cnt draws minus self-loops and coincidences with lattice or other shortcuts
+    assert 0.95 * cnt <= extra <= cnt
+
+
+def test_erdos_renyi_uniform_endpoints():
+    from oracle import gen
+    n, d = 4000, 8.0
+    _, cp, rows = gen.erdos_renyi(n, d, seed=9)
+    deg = np.diff(cp.astype(np.int64))
+    m = round(n * d / 2)
+    # undirected edges after dropping self-loops and duplicates: close to m
+    assert 0.99 * m <= deg.sum() // 2 <= m
+    # degrees follow Binomial(2m, 1/n) closely: mean d, variance about d
+    assert abs(deg.mean() - d) < 0.1
+    assert 0.8 * d < deg.var() < 1.2 * d
