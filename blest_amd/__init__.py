@@ -212,6 +212,23 @@ class BVSS(BitMatrix):
         b.slice_size, b.no_masks = i["slice_size"], i["no_masks"]
         return b
 
+    @classmethod
+    def adopt(cls, n: int, slice_size: int, no_masks: int, slice_set_ptrs, virtual_to_real, real_ptrs, row_ids,
+              masks, update_divergence: float = 0.0, full_padding: bool = False) -> "BVSS":
+        """A BVSS from the five arrays of a BVSS built elsewhere (the reference's
+        getSliceSetPtrs / getVirtualToReal / getRealPtrs / getRowIds / getMasks,
+        BVSS.cuh:54-68); copied and validated by the library."""
+        b = cls(slice_size, no_masks)
+        ssp = np.ascontiguousarray(slice_set_ptrs, dtype=np.uint64)
+        v2r, rp = _as_u32(virtual_to_real), _as_u32(real_ptrs)
+        rows, msk = _as_u32(row_ids), _as_u32(masks)
+        h = C.c_void_p()
+        check(b._lib.blest_bvss_adopt(n, slice_size, no_masks, rp.size - 1, ssp.size - 1, rows.size,
+                                      ssp.ctypes.data_as(_lib.U64P), _u32p(v2r), _u32p(rp), _u32p(rows), _u32p(msk),
+                                      float(update_divergence), int(full_padding), C.byref(h)))
+        b._h = h
+        return b
+
     def save_binary(self, path: str) -> None:
         check(self._lib.blest_bvss_save_binary(self._h, str(path).encode()))
 
@@ -251,18 +268,24 @@ def device_opts(**kw) -> _lib.DeviceOpts:
 class DeviceGraph:
     """The BVSS (and bottom-up CSR) resident in HBM; queries reuse it."""
 
-    def __init__(self, bvss: BVSS, csc: CSC | None = None, inv_perm=None, **opts):
+    def __init__(self, bvss: BVSS, csc: CSC | None = None, inv_perm=None, csr: CSC | None = None, **opts):
+        """csc: the CSC the BVSS was built from; or csr: its transpose as the reference's
+        BVSS keeps it (BVSS::getCSR(), BVSS.cuh:54,187)."""
         self._lib = _lib.load()
         self._h = C.c_void_p(0)
+        if csc is not None and csr is not None:
+            raise ValueError("pass the CSC or its transpose (csr), not both")
         o = device_opts(**opts)
-        if csc is None:
+        g = csr if csr is not None else csc
+        if g is None:
             o.keep_csr = 0
         inv = None if inv_perm is None else _as_u32(inv_perm)
         h = C.c_void_p()
-        check(self._lib.blest_device_graph_create(bvss._h, csc._h if csc is not None else None,
-                                                  _u32p(inv) if inv is not None else None, C.byref(o), C.byref(h)))
+        create = self._lib.blest_device_graph_create_csr if csr is not None else self._lib.blest_device_graph_create
+        check(create(bvss._h, g._h if g is not None else None, _u32p(inv) if inv is not None else None, C.byref(o),
+                     C.byref(h)))
         self._h = h
-        self._keep = (bvss, csc)
+        self._keep = (bvss, g)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -120,10 +120,13 @@ PROTOTYPES = {
     "blest_bvss_export": (C.c_int, [P, U64P, U32P, U32P, U32P, U32P]),
     "blest_bvss_save_binary": (C.c_int, [P, C.c_char_p]),
     "blest_bvss_load_binary": (C.c_int, [C.c_char_p, PP]),
+    "blest_bvss_adopt": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, U64P,
+                                   U32P, U32P, U32P, U32P, C.c_double, C.c_int, PP]),
     "blest_bvss_csv_header": (C.c_int, [C.c_char_p, C.c_size_t]),
     "blest_bvss_csv_row": (C.c_int, [P, C.c_char_p, C.c_size_t]),
     "blest_device_opts_default": (None, [C.POINTER(DeviceOpts)]),
     "blest_device_graph_create": (C.c_int, [P, P, U32P, C.POINTER(DeviceOpts), PP]),
+    "blest_device_graph_create_csr": (C.c_int, [P, P, U32P, C.POINTER(DeviceOpts), PP]),
     "blest_device_graph_free": (None, [P]),
     "blest_device_graph_info_get": (C.c_int, [P, C.POINTER(DeviceGraphInfo)]),
     "blest_query_create": (C.c_int, [P, C.c_uint32, PP]),

@@ -225,6 +225,34 @@ void bvss_compute_stats(BVSS& b) {
     b.compressionRatio = b.noUnpaddedSlices ? (double(edges) / double(b.noUnpaddedSlices)) / b.sliceSize : 0;
 }
 
+void bvss_validate(const BVSS& b) {
+    auto bad = [](const std::string& m) { throw Error(BLEST_ERR_INVALID_ARGUMENT, "BVSS arrays: " + m); };
+    bvss_check_shape(b.sliceSize, b.noMasks);
+    const uint64_t R = b.noRealSliceSets, V = b.noVirtualSliceSets;
+    if (R != (uint64_t(b.n) + b.sliceSize - 1) / b.sliceSize) bad("noRealSliceSets != ceil(n / sliceSize)");
+    if (b.sliceSetPtrs.size() != V + 1 || b.virtualToReal.size() != V || b.realPtrs.size() != R + 1)
+        bad("array lengths disagree with the slice-set counts");
+    if (b.noSlices % b.noMasks || b.rowIds.size() != b.noSlices || b.masks.size() != b.noSlices / b.noMasks)
+        bad("noSlices is not a whole number of mask words");
+    if (b.sliceSetPtrs[0] != 0 || b.sliceSetPtrs[V] != b.noSlices) bad("sliceSetPtrs must run from 0 to noSlices");
+    for (uint64_t v = 0; v < V; ++v) {
+        if (b.sliceSetPtrs[v + 1] < b.sliceSetPtrs[v]) bad("sliceSetPtrs decreases at vset " + std::to_string(v));
+        if (b.sliceSetPtrs[v] % b.noMasks) bad("vset " + std::to_string(v) + " does not start on a mask word");
+    }
+    if (b.realPtrs[0] != 0 || b.realPtrs[R] != V) bad("realPtrs must run from 0 to noVirtualSliceSets");
+    for (uint64_t r = 0; r < R; ++r) {
+        if (b.realPtrs[r + 1] < b.realPtrs[r]) bad("realPtrs decreases at rset " + std::to_string(r));
+        for (uint32_t v = b.realPtrs[r]; v < b.realPtrs[r + 1]; ++v)
+            if (b.virtualToReal[v] != r) bad("virtualToReal[" + std::to_string(v) + "] != its rset");
+    }
+    const uint32_t conceal = b.sliceSize == 32 ? 0xFFFFFFFFu : ((1u << b.sliceSize) - 1);
+    for (uint64_t sl = 0; sl < b.noSlices; ++sl) {
+        const uint32_t m = (b.masks[sl / b.noMasks] >> (b.sliceSize * (sl % b.noMasks))) & conceal;
+        const uint32_t row = b.rowIds[sl];
+        if (row == kNoVertex ? m != 0 : row >= b.n) bad("row id out of range at slice " + std::to_string(sl));
+    }
+}
+
 void bvss_save_binary(const BVSS& b, const std::string& path) {
     std::ofstream out(path, std::ios::binary);
     if (!out) throw Error(BLEST_ERR_IO, "cannot write BVSS cache " + path);
@@ -273,6 +301,7 @@ BVSS bvss_load_binary(const std::string& path) {
     r(b.rowIds.data(), 4 * b.rowIds.size());
     r(b.masks.data(), 4 * b.masks.size());
     if (!in) throw Error(BLEST_ERR_IO, "truncated BVSS cache " + path);
+    bvss_validate(b);
     // The cache does not record FULL_PADDING: a padded leftover vset has 32*noMasks slots.
     b.fullPadding = false;
     for (uint32_t v = 0; v < b.noVirtualSliceSets; ++v) {

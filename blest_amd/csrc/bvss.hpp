@@ -38,6 +38,11 @@ void bvss_check_shape(uint32_t sliceSize, uint32_t noMasks);
 BVSS bvss_build(const CSC& g, uint32_t sliceSize, uint32_t noMasks, bool fullPadding,
                 unsigned threads = 0);
 void bvss_compute_stats(BVSS& b);
+// Checks that b's arrays form a BVSS of the reference layout that the device can run:
+// prefix-summed sliceSetPtrs / realPtrs, virtualToReal consistent with realPtrs,
+// R = ceil(n / sliceSize), vsets of whole mask words, row IDs below n (or the pad row
+// 0xFFFFFFFF with an empty mask).  Throws BLEST_ERR_INVALID_ARGUMENT naming the fault.
+void bvss_validate(const BVSS& b);
 void bvss_save_binary(const BVSS& b, const std::string& path);  // BVSS.cuh:125-146
 BVSS bvss_load_binary(const std::string& path);                 // BVSS.cuh:148-183
 std::string bvss_csv_header();

@@ -261,6 +261,37 @@ blest_status blest_bvss_save_binary(const blest_bvss* b, const char* path) {
     });
 }
 
+blest_status blest_bvss_adopt(uint32_t n, uint32_t slice_size, uint32_t no_masks, uint32_t no_real_slice_sets,
+                              uint32_t no_virtual_slice_sets, uint64_t no_slices, const uint64_t* slice_set_ptrs,
+                              const uint32_t* virtual_to_real, const uint32_t* real_ptrs, const uint32_t* row_ids,
+                              const uint32_t* masks, double update_divergence, int full_padding, blest_bvss** out) {
+    return guarded([&] {
+        if (!out || !slice_set_ptrs || !real_ptrs) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        if ((no_virtual_slice_sets && !virtual_to_real) || (no_slices && (!row_ids || !masks)))
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        bvss_check_shape(slice_size, no_masks);
+        if (no_slices % no_masks) throw Error(BLEST_ERR_INVALID_ARGUMENT, "BVSS arrays: noSlices is not a whole number of mask words");
+        BVSS b;
+        b.n = n;
+        b.sliceSize = slice_size;
+        b.noMasks = no_masks;
+        b.noRealSliceSets = no_real_slice_sets;
+        b.noVirtualSliceSets = no_virtual_slice_sets;
+        b.noSlices = no_slices;
+        b.updateDivergence = update_divergence;
+        b.fullPadding = full_padding != 0;
+        b.sliceSetPtrs.assign(slice_set_ptrs, slice_set_ptrs + size_t(no_virtual_slice_sets) + 1);
+        b.virtualToReal.assign(virtual_to_real, virtual_to_real + no_virtual_slice_sets);
+        b.realPtrs.assign(real_ptrs, real_ptrs + size_t(no_real_slice_sets) + 1);
+        b.rowIds.assign(row_ids, row_ids + no_slices);
+        b.masks.assign(masks, masks + no_slices / no_masks);
+        bvss_validate(b);
+        bvss_compute_stats(b);
+        *out = new blest_bvss{std::move(b)};
+    });
+}
+
 blest_status blest_bvss_load_binary(const char* path, blest_bvss** out) {
     return guarded([&] {
         if (!path || !out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");

@@ -145,8 +145,14 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->use_graphs = 0;
 }
 
-blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,
-                                       const blest_device_opts* o, blest_device_graph** out) {
+}  // extern "C"
+
+// gh: the CSC the BVSS was built from (ghIsCsr = false; its transpose is the bottom-up
+// CSR unless the graph is symmetric), or that transpose itself (ghIsCsr = true, as the
+// reference's BVSS keeps it, BVSS::getCSR()).
+static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* gh, bool ghIsCsr,
+                                        const uint32_t* inv_perm, const blest_device_opts* o,
+                                        blest_device_graph** out) {
     return guarded([&] {
         if (!bh || !out) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
         const BVSS& b = bvss_of(bh);
@@ -158,7 +164,8 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
         if (o) opts = *o;
         if (opts.blocks_per_cu == 0 || opts.blocks_per_cu > 8)
             throw Error(BLEST_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 8]");
-        if (opts.keep_csr && !gh) throw Error(BLEST_ERR_INVALID_ARGUMENT, "keep_csr needs the CSC");
+        if (opts.keep_csr && !gh)
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, ghIsCsr ? "keep_csr needs the CSR" : "keep_csr needs the CSC");
         if (opts.use_graphs < 0 || opts.use_graphs > 2) throw Error(BLEST_ERR_INVALID_ARGUMENT, "use_graphs must be 0, 1 or 2");
         auto* h = new blest_device_graph();
         DeviceGraph& g = h->g;
@@ -249,15 +256,17 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
             if (g.Q) BLEST_HIP(hipMemcpy(g.masks, pMasks.data(), 4 * size_t(g.Q), hipMemcpyHostToDevice));
             if (gh) {
                 const CSC& c = csc_of(gh);
-                if (c.n != b.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "CSC and BVSS disagree on n");
+                if (c.n != b.n)
+                    throw Error(BLEST_ERR_INVALID_ARGUMENT, ghIsCsr ? "CSR and BVSS disagree on n"
+                                                                    : "CSC and BVSS disagree on n");
                 g.nnz = c.nnz();
-                g.symmetric = csc_is_symmetric(c);
+                g.symmetric = csc_is_symmetric(c);  // a graph is symmetric iff its transpose is
                 if (opts.keep_csr) {
                     // Bottom-up reads in-neighbours: the CSR of the edges j -> i, i.e. the
                     // transpose of the CSC (equal to it for a symmetric graph).
                     CSC tr;
                     const CSC* csr = &c;
-                    if (!g.symmetric) {
+                    if (!ghIsCsr && !g.symmetric) {
                         tr = csc_transpose(c);
                         csr = &tr;
                     }
@@ -286,6 +295,18 @@ blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh
         }
         *out = h;
     });
+}
+
+extern "C" {
+
+blest_status blest_device_graph_create(const blest_bvss* bh, const blest_csc* gh, const uint32_t* inv_perm,
+                                       const blest_device_opts* o, blest_device_graph** out) {
+    return create_device_graph(bh, gh, false, inv_perm, o, out);
+}
+
+blest_status blest_device_graph_create_csr(const blest_bvss* bh, const blest_csc* csr, const uint32_t* inv_perm,
+                                           const blest_device_opts* o, blest_device_graph** out) {
+    return create_device_graph(bh, csr, true, inv_perm, o, out);
 }
 
 void blest_device_graph_free(blest_device_graph* dg) { delete dg; }

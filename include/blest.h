@@ -159,6 +159,18 @@ blest_status blest_bvss_info_get(const blest_bvss* b, blest_bvss_info* info);
 blest_status blest_bvss_export(const blest_bvss* b, uint64_t* slice_set_ptrs,
                                uint32_t* virtual_to_real, uint32_t* real_ptrs,
                                uint32_t* row_ids, uint32_t* masks);
+/* Adopt a BVSS built elsewhere, e.g. by the reference's BVSS::constructFromCSCMatrix:
+ * the header values and the five arrays exactly as its accessors return them
+ * (DataStructures/BVSS.cuh:54-68: getN, getSliceSize, getNoMasks, getNoRealSliceSets,
+ * getNoVirtualSliceSets, getNoSlices, getSliceSetPtrs, getVirtualToReal, getRealPtrs,
+ * getRowIds, getMasks, getUpdateDivergence; FULL_PADDING as it was built).  The arrays
+ * are copied (the caller keeps ownership) and checked: prefix sums, virtualToReal
+ * against realPtrs, R = ceil(n / slice_size), vsets of whole mask words, row IDs below n
+ * or the pad row 0xFFFFFFFF with an empty mask (BLEST_ERR_INVALID_ARGUMENT otherwise). */
+blest_status blest_bvss_adopt(uint32_t n, uint32_t slice_size, uint32_t no_masks, uint32_t no_real_slice_sets,
+                              uint32_t no_virtual_slice_sets, uint64_t no_slices, const uint64_t* slice_set_ptrs,
+                              const uint32_t* virtual_to_real, const uint32_t* real_ptrs, const uint32_t* row_ids,
+                              const uint32_t* masks, double update_divergence, int full_padding, blest_bvss** out);
 /* BVSS binary cache (BVSS.cuh:125-183): header + the five arrays. */
 blest_status blest_bvss_save_binary(const blest_bvss* b, const char* path);
 blest_status blest_bvss_load_binary(const char* path, blest_bvss** out);
@@ -194,6 +206,13 @@ void blest_device_opts_default(blest_device_opts* o);
 blest_status blest_device_graph_create(const blest_bvss* b, const blest_csc* g,
                                        const uint32_t* inv_perm, const blest_device_opts* o,
                                        blest_device_graph** out);
+/* Same, with the bottom-up CSR given as the reference's BVSS keeps it: BVSS::getCSR()
+ * (BVSS.cuh:54,187), the transpose of the CSC (column i lists the in-neighbours of i),
+ * wrapped with blest_csc_create.  Lets a plugin that holds only the reference BitMatrix
+ * upload it without the CSC it was built from. */
+blest_status blest_device_graph_create_csr(const blest_bvss* b, const blest_csc* csr,
+                                           const uint32_t* inv_perm, const blest_device_opts* o,
+                                           blest_device_graph** out);
 void blest_device_graph_free(blest_device_graph* dg);
 typedef struct blest_device_graph_info {
     uint32_t n, rsets, vsets, grid_blocks, block_size, cus, symmetric, has_csr;

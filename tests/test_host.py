@@ -315,3 +315,51 @@ def test_permutation_check(blest):
     with pytest.raises(blest.BlestError) as e:
         g.apply_permutation(bad)
     assert e.value.status == 3
+
+
+# --------------------------------------------------- adopting a BVSS built elsewhere
+@pytest.mark.parametrize("full_padding", [False, True])
+def test_bvss_adopt_roundtrip(blest, full_padding):
+    """The five arrays of a BVSS (as the reference's accessors return them) adopted back
+    give the same arrays and statistics."""
+    g = blest.CSC.rmat(10, 8, seed=4)
+    b = blest.BVSS(8, 4).construct_from_csc(g, full_padding=full_padding)
+    arr, info = b.arrays(), b.info()
+    a = blest.BVSS.adopt(g.n, 8, 4, arr["slice_set_ptrs"], arr["virtual_to_real"], arr["real_ptrs"],
+                         arr["row_ids"], arr["masks"], info["update_divergence"], full_padding)
+    for k, v in arr.items():
+        np.testing.assert_array_equal(v, a.arrays()[k], err_msg=k)
+    ai = a.info()
+    for k in ("no_slices", "no_unpadded_slices", "no_padded_slices", "edges_check", "no_real_slice_sets",
+              "no_virtual_slice_sets", "update_divergence", "full_padding"):
+        assert ai[k] == info[k], k
+    assert ai["edges_check"] == g.nnz
+
+
+def _adopt(blest, n, arr):
+    return blest.BVSS.adopt(n, 8, 4, arr["slice_set_ptrs"], arr["virtual_to_real"], arr["real_ptrs"],
+                            arr["row_ids"], arr["masks"])
+
+
+def test_bvss_adopt_rejects_malformed_arrays(blest):
+    """Each planted fault in otherwise valid arrays is refused with a message."""
+    g = blest.CSC.rmat(9, 8, seed=2)
+    good = blest.BVSS(8, 4).construct_from_csc(g).arrays()
+    _adopt(blest, g.n, good)  # accepted
+
+    def planted(key, fn):
+        a = {k: v.copy() for k, v in good.items()}
+        fn(a[key])
+        with pytest.raises(blest.BlestError) as e:
+            _adopt(blest, g.n, a)
+        assert e.value.status == 1 and "BVSS arrays" in str(e.value)
+
+    big = int(np.argmax(np.diff(good["slice_set_ptrs"].astype(np.int64))))
+    planted("row_ids", lambda x: x.__setitem__(int(np.flatnonzero(x != 0xFFFFFFFF)[0]), g.n))  # row out of range
+    planted("virtual_to_real", lambda x: x.__setitem__(len(x) - 1, x[-1] - 1))  # wrong owner
+    planted("real_ptrs", lambda x: x.__setitem__(1, x[2] + 1))  # not monotone
+    planted("slice_set_ptrs", lambda x: x.__setitem__(big + 1, x[big + 1] - 2))  # vset not on a mask word
+    with pytest.raises(blest.BlestError):  # R != ceil(n / 8)
+        _adopt(blest, g.n + 64, good)
+    with pytest.raises(blest.BlestError):  # invalid slice size (BVSS.cuh:103-110)
+        blest.BVSS.adopt(g.n, 8, 3, *good.values())
