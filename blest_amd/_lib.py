@@ -135,6 +135,7 @@ PROTOTYPES = {
     "blest_bfs_async": (C.c_int, [P, C.c_uint32, P]),
     "blest_query_sync": (C.c_int, [P, C.POINTER(BfsStats)]),
     "blest_query_levels_device": (P, [P]),
+    "blest_bfs_noalloc": (C.c_int, [P, C.c_uint32, I32P, C.POINTER(BfsStats), P]),
     "blest_bfs_trace": (C.c_int, [P, C.c_uint32, I32P, C.POINTER(BfsStats), C.POINTER(LevelTrace), C.c_uint32,
                                   U32P]),
     "blest_mbfs": (C.c_int, [P, U32P, C.c_uint32, I32P, C.POINTER(MbfsOpts), C.POINTER(BfsStats)]),

@@ -266,6 +266,20 @@ blest_status blest_query_sync(blest_query* q, blest_bfs_stats* st);
 /* Device pointer (int32*) of the query's level array (n * max_sources entries). */
 void* blest_query_levels_device(blest_query* q);
 
+/* Single-source BFS that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code that must not call hipMalloc, such as
+ * tight serving loops or harnesses that forbid it).  Same traversal and result as
+ * blest_bfs, without a blest_query: the traversal state lives in a 64 MiB device array
+ * that the library holds on each GPU it runs on until the process ends, and the kernels
+ * run on `stream` (hipStream_t, NULL = the null stream).  levels: device memory of the
+ * graph's device, n int32 entries in original IDs (-1 = unreachable).  A graph whose
+ * state exceeds the array (about 4.2 M vertices and 1.1 M vsets, e.g. R-MAT scale 22,
+ * fit) is rejected with BLEST_ERR_INVALID_ARGUMENT.  Calls on one GPU share the array
+ * and are serialised by a lock held until the traversal has finished; the call returns
+ * when the levels are written. */
+blest_status blest_bfs_noalloc(const blest_device_graph* dg, uint32_t src, int32_t* levels,
+                               blest_bfs_stats* st, void* stream);
+
 /* Multi-source BFS (MBFS/MBFSKernel.cuh:21-53; the reference kernel file is
  * empty): k independent traversals in one launch, k <= max_sources of q.
  * levels[k*n] (row r = root r). */
