@@ -320,12 +320,12 @@ def batch_bench(a) -> dict:
 
     import blest_amd as B
     lib = B._lib.load()
-    rng = np.random.default_rng(a.batch_seed)
+    from blest_amd import workloads as W
     count = a.batch_graphs
-    sizes = rng.integers(1024, 65537, size=count)
     t0 = time.time()
-    graphs = [B.CSC.erdos_renyi(int(n), 8.0, seed=10_000 + i) for i, n in enumerate(sizes)]
-    roots = np.array([int(g.pick_roots(1, seed=i)[0]) for i, g in enumerate(graphs)], dtype=np.uint32)
+    graphs, roots = W.er_batch(count, a.batch_seed)
+    sizes = np.array([g.n for g in graphs], dtype=np.int64)
+    rng = np.random.default_rng(a.batch_seed + 1)
     log(f"[batch] {count} graphs, {int(sizes.sum())} vertices generated in {time.time() - t0:.0f}s")
     t0 = time.time()
     gb = B.GraphBatch(graphs)

@@ -10,6 +10,7 @@ Workloads (sizes as named; every seed fixed):
   grid4096_rcm     4096x4096 grid, keep 0.9, 1% shortcuts, RCM, 64 roots
   rmat26           R-MAT scale 26, edge factor 16, natural order, 64 roots
   rmat26_mbfs      R-MAT scale 26, edge factor 16, natural order, 1,024 roots (MBFS)
+  er_batch         4,096 Erdos-Renyi graphs, n ~ U[1024, 65536], degree 8, a root each
 """
 from __future__ import annotations
 
@@ -107,6 +108,17 @@ def reorder_cached(g: CSC, w: Workload, log=print) -> np.ndarray:
                                 "reorder": w.reorder.name, "window": w.window, "seconds": time.time() - t}))
     log(f"[prep] reordered in {time.time() - t:.1f}s, cached as {binp.name}")
     return inv
+
+
+def er_batch(count: int = 4096, seed: int = 3, lo: int = 1024, hi: int = 65536, avg_degree: float = 8.0):
+    """The many-small-graphs workload: `count` Erdos-Renyi graphs with n drawn uniformly
+    from [lo, hi] (seeded), average degree avg_degree, one seeded root each (a vertex of
+    degree >= 1).  Returns (graphs, roots as uint32)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, size=count)
+    graphs = [CSC.erdos_renyi(int(n), avg_degree, seed=10_000 + i) for i, n in enumerate(sizes)]
+    roots = np.array([int(g.pick_roots(1, seed=i)[0]) for i, g in enumerate(graphs)], dtype=np.uint32)
+    return graphs, roots
 
 
 @dataclass

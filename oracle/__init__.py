@@ -6,6 +6,7 @@ never imports, links or calls it.
 
 Contents and what pins each function (see tests/test_oracle.py):
   bfs            textbook FIFO BFS            pinned by scipy.sparse.csgraph BFS + hand graphs
+  bfs_many       `bfs` from many sources on host threads (same routine)
   cc             union-find, min-ID labels    pinned by scipy connected_components + hand graphs
   bvss_build     BVSS.cuh:185-467             pinned by decode == CSC, popcount == nnz, hand fixtures
   bvss_bfs       BFS/BVSSBFSKernel.cuh level replay + byte model
@@ -84,6 +85,21 @@ def bfs(n: int, col_ptrs, rows, src: int) -> np.ndarray:
     load().oracle_bfs(n, _p32(cp), _p32(rw) if rw.size else _p32(np.zeros(1, np.uint32)), src,
                       lv.ctypes.data_as(I32P))
     return lv
+
+
+def bfs_many(n: int, col_ptrs, rows, srcs, workers: int = 8):
+    """`bfs` from every source, `workers` at a time on host threads (the C routine runs
+    without the GIL).  Yields (src, levels) in the order of srcs."""
+    from concurrent.futures import ThreadPoolExecutor
+    cp, rw = _u32(col_ptrs), _u32(rows)
+    srcs = [int(s) for s in srcs]
+    workers = max(1, workers)
+    with ThreadPoolExecutor(workers) as ex:
+        for i in range(0, len(srcs), workers):  # at most `workers` level arrays held at once
+            part = srcs[i:i + workers]
+            futs = [ex.submit(bfs, n, cp, rw, s) for s in part]
+            for s, f in zip(part, futs):
+                yield s, f.result()
 
 
 def cc(n: int, col_ptrs, rows) -> tuple[np.ndarray, int]:

@@ -6,8 +6,8 @@ bit-exactly with the oracle on the original graph: BFS levels in original IDs (-
 unreachable) for the benchmark's own roots, canonical CC labels, and multi-source rows.
 R-MAT 22 under Gorder+Jaccard needs its cached ordering (computing it takes far longer
 than a test may); without the cache those cases are skipped."""
-import os
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -38,8 +38,8 @@ def device_graph(blest, p, **opts):
     return blest.DeviceGraph(p.bvss, p.csc, p.inv_perm, **opts)
 
 
-# workload -> roots checked against the oracle (the first of the benchmark's roots)
-BFS_CASES = [("rmat16_natural", 16), ("rmat22_gorder", 6), ("grid4096_rcm", 4)]
+# workload -> roots checked against the oracle (all of the benchmark's roots)
+BFS_CASES = [("rmat16_natural", 16), ("rmat22_gorder", 64), ("grid4096_rcm", 64)]
 
 
 @pytest.mark.parametrize("name,count", BFS_CASES)
@@ -50,10 +50,10 @@ def test_bfs_benchmark_roots(blest, oracle, name, count):
     cp, rows = p.orig_col_ptrs, p.orig_rows
     deg = np.diff(cp.astype(np.int64))
     k = blest.BVSSBFSKernel(device_graph(blest, p))
-    for src in [int(r) for r in p.roots[:count]]:
+    roots = [int(r) for r in p.roots[:count]]
+    for src, want in oracle.bfs_many(n, cp, rows, roots, workers=8):
         say(f"  root {src}")
         res = k.run(src)
-        want = oracle.bfs(n, cp, rows, src)
         np.testing.assert_array_equal(res.levels, want, err_msg=f"{name} src={src}")
         assert res.no_visited == int((want >= 0).sum())
         assert res.edges_in_component == deg[want >= 0].sum() // 2
@@ -71,7 +71,7 @@ def test_cc_benchmark_graph(blest, oracle, name):
 
 def test_mbfs_128_roots_rmat22(blest, oracle):
     """One 128-root launch on R-MAT 22: every row equals the single-source GPU BFS of its
-    root, and 6 rows (spread over both 64-bit halves of the batch) equal the oracle."""
+    root, and 16 rows (spread over both 64-bit halves of the batch) equal the oracle."""
     say("case workload mbfs rmat22_gorder 128 roots")
     p = workload("rmat22_gorder")
     n = p.csc.n
@@ -82,37 +82,72 @@ def test_mbfs_128_roots_rmat22(blest, oracle):
     for j, r in enumerate(roots):
         np.testing.assert_array_equal(res.level_arrays[j], single.run(int(r)).levels,
                                       err_msg=f"row {j} root {r}: multi-source vs single-source")
-    for j in (0, 31, 63, 64, 100, 127):
-        np.testing.assert_array_equal(res.level_arrays[j], oracle.bfs(n, p.orig_col_ptrs, p.orig_rows, int(roots[j])),
-                                      err_msg=f"row {j} root {roots[j]}: multi-source vs oracle")
+    rows_checked = [0, 9, 18, 31, 40, 52, 63, 64, 71, 80, 95, 100, 111, 119, 126, 127]
+    for j, (r, want) in zip(rows_checked, oracle.bfs_many(n, p.orig_col_ptrs, p.orig_rows,
+                                                          [int(roots[j]) for j in rows_checked])):
+        np.testing.assert_array_equal(res.level_arrays[j], want, err_msg=f"row {j} root {r}: multi-source vs oracle")
 
 
-@pytest.mark.skipif(os.environ.get("BLEST_LARGE_TESTS") != "1",
-                    reason="R-MAT 26 (67 M vertices, 2.1 B edges) takes minutes; set BLEST_LARGE_TESTS=1")
-def test_rmat26_bfs_mbfs_cc(blest, oracle):
-    """R-MAT 26, natural order, as benchmarked: BFS from 2 of the benchmark's roots and CC
-    against the oracle, and a 128-root multi-source launch (holding those 2 roots) whose
-    every row equals the single-source GPU BFS of its root."""
-    say("case workload rmat26 (large)")
-    p = workload("rmat26")
+def test_rmat26_bfs_cc_and_1024_root_mbfs(blest, oracle):
+    """R-MAT 26 (67 M vertices, 2.1 B directed edges), natural order, as benchmarked:
+    BFS from 4 of the benchmark's roots and CC against the oracle, then the benchmark's
+    1,024 roots as 8 launches of 128: in every launch one row of each 64-bit half equals
+    the oracle (16 rows in all), and every row reaches exactly its root's component
+    (vertex count against the oracle's component sizes)."""
+    say("case workload rmat26 bfs / cc / 1024-root mbfs")
+    t0 = time.time()
+    p = workload("rmat26_mbfs")
     n = p.csc.n
     cp, rows = p.orig_col_ptrs, p.orig_rows
-    single = blest.BVSSBFSKernel(device_graph(blest, p))
-    roots = [int(r) for r in p.roots[:64]]
-    for src in roots[:2]:
+    say(f"  prepared in {time.time() - t0:.0f}s")
+    dg = device_graph(blest, p)
+    single = blest.BVSSBFSKernel(dg)
+    roots = [int(r) for r in p.roots]
+    assert len(roots) == 1024
+    for src, want in oracle.bfs_many(n, cp, rows, roots[:4], workers=4):
         say(f"  bfs root {src}")
-        np.testing.assert_array_equal(single.run(src).levels, oracle.bfs(n, cp, rows, src), err_msg=f"src={src}")
-    say("  cc")
-    res = blest.BVSSCCKernel(p.bvss, p.csc, p.inv_perm).run()
+        np.testing.assert_array_equal(single.run(src).levels, want, err_msg=f"src={src}")
+    del single
+    say(f"  cc ({time.time() - t0:.0f}s)")
+    res = blest.BVSSCCKernel(dg).run()
     labels, count = oracle.cc(n, cp, rows)
     np.testing.assert_array_equal(res.components, labels)
     assert res.no_components == count
-    del labels, res
-    # 128 roots, as benchmarked: the benchmark's 64 and 64 more (two words per vertex)
-    extra = [int(r) for r in blest.CSC.from_arrays(n, cp, rows).pick_roots(192, seed=5) if int(r) not in set(roots)]
-    batch = roots + extra[:128 - len(roots)]
-    say(f"  mbfs {len(batch)} roots")
-    m = blest.BVSSMBFSKernel(p.bvss, p.csc, p.inv_perm, batch=128).run(np.asarray(batch, dtype=np.uint32))
-    assert len(m.level_arrays) == 128
-    for j, r in enumerate(batch):
-        np.testing.assert_array_equal(m.level_arrays[j], single.run(r).levels, err_msg=f"row {j} root {r}")
+    del res
+    comp_size = np.bincount(labels, minlength=n)
+    mb = blest.BVSSMBFSKernel(dg, batch=128)
+    for b in range(8):
+        batch = np.asarray(roots[128 * b:128 * (b + 1)], dtype=np.uint32)
+        lv = mb.run(batch).level_arrays
+        picks = [(7 * b + 3) % 64, 64 + (11 * b + 5) % 64]
+        for j, (r, want) in zip(picks, oracle.bfs_many(n, cp, rows, [int(batch[j]) for j in picks], workers=2)):
+            np.testing.assert_array_equal(lv[j], want, err_msg=f"launch {b} row {j} root {r}")
+        for j, r in enumerate(batch):
+            assert np.count_nonzero(lv[j] >= 0) == comp_size[labels[r]], f"launch {b} row {j} root {r}"
+        say(f"  mbfs launch {b}: rows {picks} equal the oracle ({time.time() - t0:.0f}s)")
+        del lv
+
+
+def test_graph_batch_er4096(blest, oracle):
+    """The many-small-graphs workload at full size: 4,096 Erdos-Renyi graphs, n ~ U[1024,
+    65536], average degree 8, one root each, all in one launch; every graph's levels
+    equal the oracle's."""
+    from blest_amd import workloads
+    say("case workload er_batch 4096 graphs")
+    t0 = time.time()
+    graphs, roots = workloads.er_batch(4096, 3)
+    gb = blest.GraphBatch(graphs)
+    levels, secs = gb.bfs(roots)
+    assert secs > 0 and len(levels) == 4096
+    say(f"  one launch of {sum(g.n for g in graphs)} vertices ({time.time() - t0:.0f}s)")
+    from concurrent.futures import ThreadPoolExecutor
+
+    def check(i):
+        cp, rows = graphs[i].arrays()
+        return np.array_equal(levels[i], oracle.bfs(graphs[i].n, cp, rows, int(roots[i])))
+
+    with ThreadPoolExecutor(8) as ex:
+        ok = list(ex.map(check, range(len(graphs))))
+    bad = [i for i, x in enumerate(ok) if not x]
+    assert not bad, f"graphs {bad[:10]} differ from the oracle"
+    say(f"  4096 graphs equal the oracle ({time.time() - t0:.0f}s)")
