@@ -17,7 +17,11 @@ carries:
                 levels and direction schedules) / those launches' mean time, against
                 8 TB/s;
   cpu_baseline  the oracle's level-synchronous BVSS BFS (the reference algorithm,
-                its own direction rule) on all host cores over a bounded root sample.
+                its own direction rule) on all host cores over a bounded root sample;
+  cc            connected components of the same resident graph;
+  mbfs          batched multi-source BFS (128 roots per launch, 1,024 seeded roots) on
+                the same resident graph: aggregate GTEPS, the plain single-source GTEPS
+                of the same roots and the ratio of the two.
 """
 from __future__ import annotations
 
@@ -260,14 +264,16 @@ def serve_bench(a) -> dict:
             "rates": rows, "roofline": None, "cpu_baseline": None}
 
 
-def mbfs_bench(a, prep, opts) -> dict:
+def mbfs_bench(a, prep, opts, dg=None) -> dict:
     """Batched multi-source BFS against plain single-source runs on the same roots:
-    aggregate GTEPS = sum of m_c / sum of kernel time; speedup = batched / plain."""
+    aggregate GTEPS = sum of m_c / sum of kernel time; speedup = batched / plain.
+    dg: the resident device graph to use (uploaded with opts when None)."""
     import ctypes as C
 
     import blest_amd as B
     lib = B._lib.load()
-    dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
+    if dg is None:
+        dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
     roots = [int(r) for r in prep.roots]
     if a.mbfs_roots > len(roots):
         # more seeded roots of degree >= 1, picked on the resident (reordered) graph and
@@ -417,6 +423,7 @@ def main():
     ap.add_argument("--serve-buffers", type=int, default=48, help="serve: device result buffers per graph")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--mbfs", action="store_true", help="batched multi-source BFS vs plain on the workload's roots")
+    ap.add_argument("--no-mbfs", action="store_true", help="skip the multi-source summary of the default line")
     ap.add_argument("--mbfs-width", type=int, default=128)
     ap.add_argument("--mbfs-roots", type=int, default=1024)
     ap.add_argument("--batch-graphs", type=int, default=4096)
@@ -550,10 +557,19 @@ def main():
                 ach = by_dir[d][0] / by_dir[d][1] / 1e9
                 directions[name] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                                     "ms_per_launch": round(by_dir[d][1] * 1e3 / len(replay_roots), 4)}
+        # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # summary of this workload, used only when it was taken at the schedule this run
+        # uses (launch shape and switch thresholds); otherwise null
         traffic = None
         tf = ROOT / "profiles" / f"bfs_kernel_{a.workload}_traffic.json"
         if tf.exists():
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            sched = tj.get("schedule") or {}
+            if all(abs(float(sched.get(k, -1)) - float(opts.get(k, dflt))) < 1e-9
+                   for k, dflt in (("blocks_per_cu", 4), ("alpha", 15.0), ("beta", 18.0))):
+                traffic = tj.get("bytes_per_launch")
+            else:
+                log(f"[bench] traffic summary {tf.name} was taken at {sched}, not at this run's schedule: null")
         used_ms = float(np.mean([s_[1] for s_ in used]))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -600,6 +616,14 @@ def main():
         cc = {"ms": round(float(np.mean(cc_ms)), 4), "components": cs.no_components,
               "medges_per_s": round(m / (np.mean(cc_ms) * 1e-3) / 1e6, 1), "algo": "hook+compress"}
         log(f"[bench] cc: {cc}")
+    mb = None
+    if not a.no_mbfs and info_sym(dg):
+        # batched multi-source BFS against plain single-source BFS on the same resident
+        # graph: 128-root launches over 1,024 seeded roots
+        r = mbfs_bench(a, prep, opts, dg)
+        mb = {"gteps": r["value"], "plain_gteps": r["plain_gteps"], "speedup_vs_plain": r["speedup_vs_plain"],
+              "roots": r["config"]["roots"], "batch": r["config"]["batch"], "batch_ms": r["batch_ms"],
+              "plain_ms_per_root": r["plain_ms_per_root"], "algo": "bit-parallel"}
     info = dg.info()
     cfg = {"workload": w.name}
     if w.kind == "rmat":
@@ -617,7 +641,7 @@ def main():
     line = {"metric": f"BFS GTEPS (harmonic mean, {len(roots)} roots)", "value": round(value, 3), "unit": "GTEPS",
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
-            "config": cfg, "roofline": roofline, "cpu_baseline": cpu, "cc": cc}
+            "config": cfg, "roofline": roofline, "cpu_baseline": cpu, "cc": cc, "mbfs": mb}
     print(json.dumps(line), flush=True)
 
 

@@ -27,6 +27,9 @@ from . import BVSS, CSC, Reorder
 
 ROOT = Path(__file__).resolve().parent.parent
 CACHE = Path(os.environ.get("BLEST_CACHE", ROOT / "bench_cache"))
+# Version of the ordering algorithms (reorder.cpp) a cached ordering was computed with:
+# raised whenever a change could alter an order, so that no stale ordering is reused.
+ORDERING_VERSION = 2
 
 
 @dataclass
@@ -81,7 +84,7 @@ def csc_digest(g: CSC) -> str:
 
 
 def _cache_paths(w: Workload, digest: str) -> tuple[Path, Path]:
-    stem = f"{w.name}_{int(w.reorder)}_{w.window}_{digest[:16]}"
+    stem = f"{w.name}_{int(w.reorder)}_{w.window}_v{ORDERING_VERSION}_{digest[:16]}"
     return CACHE / f"{stem}_ordering.bin", CACHE / f"{stem}_ordering.json"
 
 
@@ -93,7 +96,7 @@ def reorder_cached(g: CSC, w: Workload, log=print) -> np.ndarray:
     binp, meta = _cache_paths(w, digest)
     if binp.exists() and meta.exists():
         info = json.loads(meta.read_text())
-        if info.get("digest") == digest and info.get("n") == g.n:
+        if info.get("digest") == digest and info.get("n") == g.n and info.get("version") == ORDERING_VERSION:
             inv = np.fromfile(binp, dtype=np.uint32)
             if inv.size == g.n:
                 g.apply_permutation(inv)
@@ -104,7 +107,7 @@ def reorder_cached(g: CSC, w: Workload, log=print) -> np.ndarray:
     inv = g.reorder(w.reorder, window=w.window)
     CACHE.mkdir(parents=True, exist_ok=True)
     inv.astype(np.uint32).tofile(binp)
-    meta.write_text(json.dumps({"digest": digest, "n": g.n, "nnz": g.nnz, "workload": w.name,
+    meta.write_text(json.dumps({"digest": digest, "version": ORDERING_VERSION, "n": g.n, "nnz": g.nnz, "workload": w.name,
                                 "reorder": w.reorder.name, "window": w.window, "seconds": time.time() - t}))
     log(f"[prep] reordered in {time.time() - t:.1f}s, cached as {binp.name}")
     return inv

@@ -13,6 +13,7 @@ Contents and what pins each function (see tests/test_oracle.py):
                                               pinned by equality with `bfs` on every graph tested
   bvss_cc        CC/BVSSCCKernel.cuh label propagation
                                               pinned by equality with `cc`
+  jaccard_window CSC.cuh:717-920 restated     pinned by hand-computed orders (tests/test_oracle.py)
   gen.*          numpy restatement of the generators (pins blest's C++ generators bit-exactly)
   social.*       Common.cuh:100-250 restated  pinned by closed-form degree sequences
   gorder_ref     the reference's own Gorder compiled into oracle/_ref (when built)
@@ -67,6 +68,8 @@ def load():
         lib.oracle_bvss_cc.restype = C.c_uint32
         lib.oracle_bvss_cc.argtypes = [C.c_uint32, C.c_uint32, U64P, U32P, U32P, U32P, U32P, U32P]
         lib.oracle_max_threads.restype = C.c_int
+        lib.oracle_jaccard_window.restype = None
+        lib.oracle_jaccard_window.argtypes = [C.c_uint32, U32P, U32P, C.c_uint32, C.c_uint32, U32P]
         _lib = lib
     return _lib
 
@@ -161,6 +164,15 @@ def bvss_cc(n: int, bv: dict) -> tuple[np.ndarray, int]:
                             _p32(bv["real_ptrs"]), _p32(bv["row_ids"]) if len(bv["row_ids"]) else _p32(e),
                             _p32(bv["masks"]) if len(bv["masks"]) else _p32(e), _p32(lab))
     return lab, int(it)
+
+
+def jaccard_window(n: int, col_ptrs, rows, window: int, slice_size: int = 8) -> np.ndarray:
+    """CSC::jackardWithWindow restated: inv_perm[old] = new."""
+    cp, rw = _u32(col_ptrs), _u32(rows)
+    inv = np.empty(n, dtype=np.uint32)
+    load().oracle_jaccard_window(n, _p32(cp), _p32(rw) if rw.size else _p32(np.zeros(1, np.uint32)), slice_size,
+                                 window, _p32(inv))
+    return inv
 
 
 def max_threads() -> int:

@@ -416,3 +416,107 @@ uint32_t oracle_bvss_cc(uint32_t n, uint32_t V, const uint64_t* ssp, const uint3
     free(list2);
     return it;
 }
+
+/* ------------------------------------------------------------------ Jaccard window */
+/* Plain restatement of CSC::jackardWithWindow (DataStructures/CSC.cuh:717-920), one
+ * window after another (the reference runs windows in parallel; they are independent).
+ * Per slot of slice_size columns inside a window: the seed is the unplaced column of
+ * largest degree, the LAST such index on ties (`>=`, CSC.cuh:772-779); every further
+ * column is the queued candidate (an unplaced window column sharing a row with the
+ * slot's union) of largest inter / (union + deg - inter), the FIRST in queue order on
+ * ties (`>`, CSC.cuh:853), taken out by moving the queue's last entry into its place
+ * (CSC.cuh:860-862); with an empty queue the seed rule picks again.  A row's scan of
+ * the transpose starts at its first column >= the window start and stops at the first
+ * column >= the window end (CSC.cuh:795-826).  inv_perm[old] = new. */
+void oracle_jaccard_window(uint32_t n, const uint32_t* cp, const uint32_t* rows, uint32_t slice, uint32_t window,
+                           uint32_t* inv_perm) {
+    /* transpose: tp / ti list, for each row r, the columns c with r in column c (sorted) */
+    uint64_t nnz = cp[n];
+    uint32_t* tp = calloc((size_t)n + 1, sizeof(uint32_t));
+    uint32_t* ti = malloc((nnz ? nnz : 1) * sizeof(uint32_t));
+    for (uint64_t e = 0; e < nnz; ++e) tp[rows[e] + 1]++;
+    for (uint32_t r = 0; r < n; ++r) tp[r + 1] += tp[r];
+    uint32_t* fill = malloc(((size_t)n + 1) * sizeof(uint32_t));
+    memcpy(fill, tp, ((size_t)n + 1) * sizeof(uint32_t));
+    for (uint32_t c = 0; c < n; ++c)
+        for (uint32_t e = cp[c]; e < cp[c + 1]; ++e) ti[fill[rows[e]]++] = c;
+    free(fill);
+    const uint32_t NONE = 0xFFFFFFFFu;
+    unsigned char* placed = malloc(window ? window : 1);
+    uint32_t* rowStart = malloc(((size_t)n ? n : 1) * sizeof(uint32_t));
+    uint32_t* inter = malloc((window ? window : 1) * sizeof(uint32_t));
+    uint32_t* queue = malloc((window ? window : 1) * sizeof(uint32_t));
+    uint32_t* rowMark = calloc((size_t)n ? n : 1, sizeof(uint32_t));
+    uint32_t epoch = 1;
+    for (uint32_t ws = 0; ws < n; ws += window) {
+        const uint32_t we = ws + window < n ? ws + window : n;
+        memset(placed, 0, window);
+        for (uint32_t r = 0; r < n; ++r) rowStart[r] = NONE;
+        for (uint32_t ss = ws; ss < we; ss += slice) {
+            const uint32_t se = ss + slice < we ? ss + slice : we;
+            memset(inter, 0, window * sizeof(uint32_t));
+            uint32_t qn = 0, uni = 0;
+            ++epoch;
+            for (uint32_t newCol = ss; newCol < se; ++newCol) {
+                uint32_t col = NONE;
+                if (qn == 0) { /* seed: unplaced column of largest degree, last on ties */
+                    uint32_t best = 0;
+                    for (uint32_t j = ws; j < we; ++j)
+                        if (!placed[j - ws] && cp[j + 1] - cp[j] >= best) {
+                            col = j;
+                            best = cp[j + 1] - cp[j];
+                        }
+                    if (col == NONE) break;
+                } else { /* best Jaccard ratio in queue order, first on ties */
+                    uint32_t bi = 0;
+                    double bv = -1.0;
+                    for (uint32_t q = 0; q < qn; ++q) {
+                        const uint32_t j = queue[q];
+                        const uint32_t in = inter[j - ws], deg = cp[j + 1] - cp[j];
+                        const uint32_t un = uni + deg - in;
+                        const double v = un == 0 ? 1.0 : (double)in / un;
+                        if (v > bv) {
+                            bv = v;
+                            bi = q;
+                        }
+                    }
+                    col = queue[bi];
+                    queue[bi] = queue[qn - 1];
+                    --qn;
+                }
+                placed[col - ws] = 1;
+                /* absorb the column's rows that are new to the union */
+                for (uint32_t e = cp[col]; e < cp[col + 1]; ++e) {
+                    const uint32_t r = rows[e];
+                    if (rowMark[r] == epoch) continue;
+                    rowMark[r] = epoch;
+                    ++uni;
+                    if (rowStart[r] == NONE) {
+                        for (uint32_t p = tp[r]; p < tp[r + 1]; ++p)
+                            if (ti[p] >= ws) {
+                                rowStart[r] = p;
+                                break;
+                            }
+                        if (rowStart[r] == NONE) continue;
+                    }
+                    for (uint32_t p = rowStart[r]; p < tp[r + 1]; ++p) {
+                        const uint32_t c = ti[p];
+                        if (c >= we) break;
+                        if (!placed[c - ws]) {
+                            if (inter[c - ws] == 0) queue[qn++] = c;
+                            inter[c - ws]++;
+                        }
+                    }
+                }
+                inv_perm[col] = newCol;
+            }
+        }
+    }
+    free(tp);
+    free(ti);
+    free(placed);
+    free(rowStart);
+    free(inter);
+    free(queue);
+    free(rowMark);
+}

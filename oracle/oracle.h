@@ -65,6 +65,12 @@ uint32_t oracle_bvss_cc(uint32_t n, uint32_t no_vsets, const uint64_t* slice_set
                         const uint32_t* virtual_to_real, const uint32_t* real_ptrs,
                         const uint32_t* row_ids, const uint32_t* masks, uint32_t* labels);
 
+/* CSC::jackardWithWindow (CSC.cuh:717-920) restated plainly: inv_perm[old] = new for
+ * the greedy Jaccard fill of each slot of `slice` columns inside windows of `window`
+ * columns (window a multiple of slice). */
+void oracle_jaccard_window(uint32_t n, const uint32_t* col_ptrs, const uint32_t* rows, uint32_t slice,
+                           uint32_t window, uint32_t* inv_perm);
+
 /* Number of OpenMP threads a parallel region gets with threads = 0. */
 int oracle_max_threads(void);
 

@@ -27,7 +27,7 @@ def workload(name):
     if name not in _prepared:
         w = workloads.WORKLOADS[name]
         if w.reorder.name == "GORDER_JACCARD":
-            stem = f"{w.name}_{int(w.reorder)}_{w.window}_"
+            stem = f"{w.name}_{int(w.reorder)}_{w.window}_v{workloads.ORDERING_VERSION}_"
             if not any(workloads.CACHE.glob(stem + "*_ordering.bin")):
                 pytest.skip(f"{name}: no cached ordering under {workloads.CACHE}")
         _prepared[name] = workloads.prepare(name, keep_original=True, log=say)

@@ -363,3 +363,57 @@ def test_bvss_adopt_rejects_malformed_arrays(blest):
         _adopt(blest, g.n + 64, good)
     with pytest.raises(blest.BlestError):  # invalid slice size (BVSS.cuh:103-110)
         blest.BVSS.adopt(g.n, 8, 3, *good.values())
+
+
+def _hub_graph(blest, n=5000):
+    src, dst = [], []
+    for h in range(4):
+        for v in range(n):
+            if v != h and (h == 0 or v % (h + 1) == 0):
+                src.append(h)
+                dst.append(v)
+    return blest.CSC.from_edges(n, np.array(src), np.array(dst), undirected=True)
+
+
+JACCARD_EXACT = {
+    "rmat14_w4096": (lambda b: b.CSC.rmat(14, 16, seed=3), 4096),
+    "rmat14_w1024": (lambda b: b.CSC.rmat(14, 16, seed=3), 1024),
+    "rmat12_w65536": (lambda b: b.CSC.rmat(12, 16, seed=4), 65536),
+    "rmat13_natural_w256": (lambda b: b.CSC.rmat(13, 16, seed=5, permute=False), 256),
+    "grid100_w2048": (lambda b: b.CSC.grid(100, 0.9, 0.01, 3), 2048),
+    "hubs_w1024": (_hub_graph, 1024),  # rows of thousands of columns per window
+    "er20000_w8192": (lambda b: b.CSC.erdos_renyi(20000, 12.0, seed=2), 8192),
+}
+
+
+@pytest.mark.parametrize("case", list(JACCARD_EXACT))
+def test_jaccard_order_equals_oracle(blest, oracle, case):
+    """The product's Jaccard window produces exactly the order of the plain restatement
+    of CSC::jackardWithWindow (CSC.cuh:717-920), tie rules included."""
+    make, window = JACCARD_EXACT[case]
+    g = make(blest)
+    cp, rows = g.arrays()
+    want = oracle.jaccard_window(g.n, cp, rows, window)
+    np.testing.assert_array_equal(g.reorder(blest.Reorder.JACCARD, window=window), want)
+
+
+def test_jaccard_order_equals_oracle_directed(blest, oracle):
+    rng = np.random.default_rng(1)
+    n = 3000
+    e = rng.integers(0, n, size=(20000, 2))
+    g = blest.CSC.from_edges(n, e[:, 0], e[:, 1], undirected=False)
+    cp, rows = g.arrays()
+    np.testing.assert_array_equal(g.reorder(blest.Reorder.JACCARD, window=512),
+                                  oracle.jaccard_window(n, cp, rows, 512))
+
+
+def test_gorder_jaccard_chain_equals_oracle_jaccard(blest, oracle):
+    """GORDER_JACCARD = the Jaccard window applied to the Gorder-relabelled CSC
+    (CSC.cuh:932-948): its Jaccard stage equals the restatement on that CSC."""
+    g = blest.CSC.rmat(12, 16, seed=9)
+    h = blest.CSC.from_arrays(g.n, *g.arrays())
+    inv_g = h.reorder(blest.Reorder.GORDER, window=4096)
+    cp, rows = h.arrays()
+    inv_j = oracle.jaccard_window(g.n, cp, rows, 4096)
+    chained = inv_j[inv_g]
+    np.testing.assert_array_equal(g.reorder(blest.Reorder.GORDER_JACCARD, window=4096), chained)

@@ -351,3 +351,50 @@ def test_erdos_renyi_uniform_endpoints():
     # degrees follow Binomial(2m, 1/n) closely: mean d, variance about d
     assert abs(deg.mean() - d) < 0.1
     assert 0.8 * d < deg.var() < 1.2 * d
+
+
+def _csc_of(adj, n):
+    cp, rows = [0], []
+    for j in range(n):
+        rows += sorted(adj.get(j, []))
+        cp.append(len(rows))
+    return np.array(cp, np.uint32), np.array(rows, np.uint32)
+
+
+def test_jaccard_window_hand_orders(oracle):
+    """CSC::jackardWithWindow restated, pinned by orders worked out by hand from the
+    reference's rules (CSC.cuh:717-920)."""
+    # two triangles' worth of ties: an 8-cycle 3-4-5-6-7 closed by 3-7 and a triangle
+    # 0-1-2, all degree 2, slots of 4 in one window of 8.  Slot 0: seed = the LAST
+    # unplaced column of largest degree (7); candidates 4 and 5 tie at 1/3 and the first
+    # in queue order wins (4), the last queue entry moving into its place; then 5 (1/4
+    # vs 6), then 6 (1/5 vs 3).  Slot 1: seed 3 (last of 0..3), its rows reach only
+    # placed columns, so the seed rule picks again (2), then 1 (first of the tie with 0),
+    # then 0.
+    adj = {0: [1, 2], 1: [0, 2], 2: [0, 1], 3: [4, 7], 4: [3, 5], 5: [4, 6], 6: [5, 7], 7: [3, 6]}
+    cp, rows = _csc_of(adj, 8)
+    np.testing.assert_array_equal(oracle.jaccard_window(8, cp, rows, 8, 4), [7, 6, 5, 4, 1, 2, 3, 0])
+    # two windows of 4: window 0 seeds with column 3 (degree 4), then 0 (ratio 3/4 over
+    # 1/2, 1/2), then 2 before 1 (tie; 2 moved to the front of the queue when 0 left);
+    # window 1 seeds with 6 (last of the degree-3 columns), then 7 (2/3 over 1/2), 4, 5.
+    adj = {0: [4, 5, 6], 1: [4, 5], 2: [6, 7], 3: [4, 5, 6, 7], 4: [0, 1, 3], 5: [0, 1, 3], 6: [0, 2, 3],
+           7: [2, 3]}
+    cp, rows = _csc_of(adj, 8)
+    np.testing.assert_array_equal(oracle.jaccard_window(8, cp, rows, 4, 4), [1, 3, 2, 0, 6, 7, 4, 5])
+
+
+def test_jaccard_window_is_a_windowed_bijection(oracle):
+    rng = np.random.default_rng(5)
+    n = 1000
+    e = rng.integers(0, n, size=(6000, 2))
+    g = {j: [] for j in range(n)}
+    for a, b in e:
+        if a != b:
+            g[int(a)].append(int(b))
+            g[int(b)].append(int(a))
+    g = {j: sorted(set(v)) for j, v in g.items()}
+    cp, rows = _csc_of(g, n)
+    inv = oracle.jaccard_window(n, cp, rows, 128, 8)
+    assert sorted(inv.tolist()) == list(range(n))
+    old = np.arange(n)
+    assert np.array_equal(inv // 128, old // 128)  # every column stays inside its window
