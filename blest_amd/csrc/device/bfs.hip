@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     // for the whole level.  (A fully wave-cooperative scan, lanes dealt out over the
     // word's open vertices every step, measured slower: its per-step shuffles cost as
     // much as a memory round trip.)
-    constexpr int kBuGroup = 4, kBuBatch = 4;
+    constexpr int kBuGroup = 8, kBuBatch = 4;
     constexpr uint32_t kBuBudget = 32;
     constexpr unsigned kHeavyCap = 1024;
     __shared__ uint32_t heavyU[kHeavyCap], heavyP[kHeavyCap];
@@ -243,6 +243,8 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     if (threadIdx.x == 0) heavyN = 0;
     __syncthreads();
     const uint32_t ws = blockIdx.x * a.sv.wps, we = min(g.W, ws + a.sv.wps);
+    // frontier tests read the 32-bit half of the bitmap word that holds the bit
+    const uint32_t* fin32 = reinterpret_cast<const uint32_t*>(fin);
     // wave-uniform (readfirstlane): the group's word indices live in scalar registers
     const unsigned wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), wpb = blockDim.x / kWave;
     unsigned long long news = 0, edges = 0;
@@ -260,21 +262,40 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             if (pe[k] > pb[k]) cand |= 1u << k;
         }
         unsigned found = 0;
+        // The first in-neighbour of every open vertex of the group at once (all index
+        // loads, then all frontier words): most vertices of a bottom-up level hit there,
+        // so the group costs one chain of dependent loads, not one per word.  A vertex
+        // still open afterwards goes on from its second entry (pb stays the list start:
+        // its degree is counted when it is found).
+        {
+            uint32_t nb0[kBuGroup], fw0[kBuGroup];
+#pragma unroll
+            for (int j = 0; j < kBuGroup; ++j) nb0[j] = ((cand >> j) & 1u) ? g.csrIdx[pb[j]] : 0u;
+#pragma unroll
+            for (int j = 0; j < kBuGroup; ++j) fw0[j] = ((cand >> j) & 1u) ? fin32[nb0[j] >> 5] : 0u;
+#pragma unroll
+            for (int j = 0; j < kBuGroup; ++j) {
+                if ((cand >> j) & 1u) {
+                    const bool hit0 = (fw0[j] >> (nb0[j] & 31)) & 1u;
+                    if (hit0) found |= 1u << j;
+                    if (hit0 || pb[j] + 1 >= pe[j]) cand &= ~(1u << j);
+                }
+            }
+        }
         int k = cand ? __builtin_ctz(cand) : kBuGroup;
-        uint32_t p = pick(pb, k), p1 = pick(pe, k);
+        uint32_t p = pick(pb, k) + 1, p1 = pick(pe, k);
         uint32_t budget = p + kBuBudget;
-        unsigned step = 1;
+        unsigned step = 2;
         while (k < kBuGroup) {  // p < p1 here
             const uint32_t lim = min(p + step, p1);
-            uint32_t nb[kBuBatch];
-            unsigned long long fw[kBuBatch];
+            uint32_t nb[kBuBatch], fw[kBuBatch];
 #pragma unroll
             for (int i = 0; i < kBuBatch; ++i) nb[i] = p + i < lim ? g.csrIdx[p + i] : 0u;
 #pragma unroll
-            for (int i = 0; i < kBuBatch; ++i) fw[i] = p + i < lim ? fin[nb[i] >> 6] : 0ull;
+            for (int i = 0; i < kBuBatch; ++i) fw[i] = p + i < lim ? fin32[nb[i] >> 5] : 0u;
             bool hit = false;
 #pragma unroll
-            for (int i = 0; i < kBuBatch; ++i) hit |= (fw[i] >> (nb[i] & 63)) & 1ull;
+            for (int i = 0; i < kBuBatch; ++i) hit |= (fw[i] >> (nb[i] & 31)) & 1u;
             p = lim;
             step = min(2 * step, unsigned(kBuBatch));
             bool next = hit || p >= p1;
@@ -292,10 +313,10 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                 if (hit) found |= 1u << k;
                 cand &= ~(1u << k);
                 k = cand ? __builtin_ctz(cand) : kBuGroup;
-                p = pick(pb, k);
+                p = pick(pb, k) + 1;
                 p1 = pick(pe, k);
                 budget = p + kBuBudget;
-                step = 1;
+                step = 2;
             }
         }
 #pragma unroll
