@@ -264,10 +264,12 @@ def serve_bench(a) -> dict:
             "rates": rows, "roofline": None, "cpu_baseline": None}
 
 
-def mbfs_bench(a, prep, opts, dg=None) -> dict:
+def mbfs_bench(a, prep, opts, dg=None, algos=(0,)) -> dict:
     """Batched multi-source BFS against plain single-source runs on the same roots:
     aggregate GTEPS = sum of m_c / sum of kernel time; speedup = batched / plain.
-    dg: the resident device graph to use (uploaded with opts when None)."""
+    dg: the resident device graph to use (uploaded with opts when None).  algos: the
+    multi-source products timed (0 bit-parallel VALU, 1 i8 MFMA); the first is the
+    headline, the others are reported beside it."""
     import ctypes as C
 
     import blest_amd as B
@@ -292,19 +294,28 @@ def mbfs_bench(a, prep, opts, dg=None) -> dict:
         mc.append(st.edges_traversed_component)
     del q1
     qm = B._Query(dg, width)
-    o = B._lib.MbfsOpts(0, width)
     batches = [np.ascontiguousarray(np.array(roots[i:i + width], dtype=np.uint32)) for i in range(0, len(roots), width)]
-    B.check(lib.blest_mbfs(qm._h, B._u32p(batches[0]), len(batches[0]), None, C.byref(o), C.byref(st)))  # warm-up
-    batch_ms = []
-    for _ in range(a.steps):
-        for b in batches:
-            B.check(lib.blest_mbfs(qm._h, B._u32p(b), len(b), None, C.byref(o), C.byref(st)))
-            batch_ms.append(st.kernel_ms)
     total_mc = float(sum(mc))
     plain = total_mc / (sum(plain_ms) * 1e-3) / 1e9
+    per_algo = {}
+    for algo in algos:
+        o = B._lib.MbfsOpts(algo, width)
+        B.check(lib.blest_mbfs(qm._h, B._u32p(batches[0]), len(batches[0]), None, C.byref(o), C.byref(st)))  # warm-up
+        ms = []
+        for _ in range(a.steps):
+            for b in batches:
+                B.check(lib.blest_mbfs(qm._h, B._u32p(b), len(b), None, C.byref(o), C.byref(st)))
+                ms.append(st.kernel_ms)
+        per_algo[algo] = ms
+        gt = total_mc * a.steps / (sum(ms) * 1e-3) / 1e9
+        log(f"[mbfs] algo {algo}: plain {plain:.2f} GTEPS ({np.mean(plain_ms):.3f} ms/root), batched {gt:.2f} GTEPS "
+            f"({np.mean(ms):.3f} ms per {width}-root batch): {gt / plain:.2f}x")
+    batch_ms = per_algo[algos[0]]
     batched = total_mc * a.steps / (sum(batch_ms) * 1e-3) / 1e9
-    log(f"[mbfs] plain {plain:.2f} GTEPS ({np.mean(plain_ms):.3f} ms/root), batched {batched:.2f} GTEPS "
-        f"({np.mean(batch_ms):.3f} ms per {width}-root batch): {batched / plain:.2f}x")
+    others = {str(k): {"gteps": round(total_mc * a.steps / (sum(v) * 1e-3) / 1e9, 3),
+                       "batch_ms": round(float(np.mean(v)), 4),
+                       "speedup_vs_plain": round(total_mc * a.steps / (sum(v) * 1e-3) / 1e9 / plain, 2)}
+              for k, v in per_algo.items()}
     info = dg.info()
     return {"metric": f"batched MBFS aggregate GTEPS ({width} roots per launch)", "value": round(batched, 3),
             "unit": "GTEPS", "n_gpus": 1, "steps": a.steps, "warmup": 1,
@@ -314,7 +325,8 @@ def mbfs_bench(a, prep, opts, dg=None) -> dict:
                        "roots": len(roots), "batch": width, "reorder": prep.workload.reorder.name.lower()},
             "plain_gteps": round(plain, 3), "speedup_vs_plain": round(batched / plain, 2),
             "plain_ms_per_root": round(float(np.mean(plain_ms)), 4),
-            "batch_ms": round(float(np.mean(batch_ms)), 4), "roofline": None, "cpu_baseline": None}
+            "batch_ms": round(float(np.mean(batch_ms)), 4), "by_algo": others, "roofline": None,
+            "cpu_baseline": None}
 
 
 def batch_bench(a) -> dict:
@@ -406,6 +418,8 @@ def main():
     ap.add_argument("--alpha", type=float, default=0.0)
     ap.add_argument("--beta", type=float, default=0.0)
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
+    ap.add_argument("--td-quads", type=int, default=0, help="top-down slice tile: quads per lane per step (1, 2, 4)")
+    ap.add_argument("--bu-group", type=int, default=0, help="bottom-up staging depth: words probed together (4, 8)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick blocks_per_cu / alpha / beta with the native autotuner on --autotune-roots "
                          "of the workload's roots before timing (default on; --no-autotune keeps the "
@@ -425,6 +439,8 @@ def main():
     ap.add_argument("--mbfs", action="store_true", help="batched multi-source BFS vs plain on the workload's roots")
     ap.add_argument("--no-mbfs", action="store_true", help="skip the multi-source summary of the default line")
     ap.add_argument("--mbfs-width", type=int, default=128)
+    ap.add_argument("--mbfs-algo", type=lambda x: [int(v) for v in x.split(",")], default=[0],
+                    help="multi-source products to time: 0 bit-parallel VALU, 1 i8 MFMA (comma list)")
     ap.add_argument("--mbfs-roots", type=int, default=1024)
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-seed", type=int, default=3)
@@ -467,18 +483,22 @@ def main():
     if a.use_graphs >= 0:
         opts["use_graphs"] = a.use_graphs
     if a.mbfs:
-        print(json.dumps(mbfs_bench(a, prep, opts)), flush=True)
+        print(json.dumps(mbfs_bench(a, prep, opts, algos=tuple(a.mbfs_algo))), flush=True)
         return
     tuned = None
-    if a.autotune and not (a.blocks_per_cu or a.alpha or a.beta):
+    if a.td_quads:
+        opts["td_quads"] = a.td_quads
+    if a.bu_group:
+        opts["bu_group"] = a.bu_group
+    if a.autotune and not (a.blocks_per_cu or a.alpha or a.beta or a.td_quads or a.bu_group):
         # launch shape and switch thresholds chosen by the native autotuner on some of the roots
         t0 = time.time()
         best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:a.autotune_roots],
                                                **opts)
-        for k in ("blocks_per_cu", "alpha", "beta"):
+        for k in ("blocks_per_cu", "td_quads", "bu_group", "alpha", "beta"):
             opts[k] = best[k]
-        tuned = {"blocks_per_cu": best["blocks_per_cu"], "alpha": best["alpha"], "beta": best["beta"],
-                 "trials": trials, "tuning_ms_per_bfs": round(best_ms, 4)}
+        tuned = {"blocks_per_cu": best["blocks_per_cu"], "td_quads": best["td_quads"], "bu_group": best["bu_group"],
+                 "alpha": best["alpha"], "beta": best["beta"], "trials": trials, "tuning_ms_per_bfs": round(best_ms, 4)}
         log(f"[bench] autotuned in {time.time() - t0:.1f}s: {tuned}")
     t0 = time.time()
     dg = B.DeviceGraph(prep.bvss, prep.csc, prep.inv_perm, **opts)
@@ -564,12 +584,14 @@ def main():
         tf = ROOT / "profiles" / f"bfs_kernel_{a.workload}_traffic.json"
         if tf.exists():
             tj = json.loads(tf.read_text())
-            sched = tj.get("schedule") or {}
-            if all(abs(float(sched.get(k, -1)) - float(opts.get(k, dflt))) < 1e-9
-                   for k, dflt in (("blocks_per_cu", 4), ("alpha", 15.0), ("beta", 18.0))):
-                traffic = tj.get("bytes_per_launch")
-            else:
-                log(f"[bench] traffic summary {tf.name} was taken at {sched}, not at this run's schedule: null")
+            mine = {k: float(opts.get(k, dflt)) for k, dflt in (("blocks_per_cu", 4), ("td_quads", 2), ("bu_group", 8),
+                                                                 ("alpha", 15.0), ("beta", 18.0))}
+            for entry in tj.get("schedules", []):
+                sched = entry.get("schedule") or {}
+                if all(abs(float(sched.get(k, -1)) - v) < 1e-9 for k, v in mine.items()):
+                    traffic = entry.get("bytes_per_launch")
+            if traffic is None:
+                log(f"[bench] no traffic summary in {tf.name} taken at this run's schedule {mine}: null")
         used_ms = float(np.mean([s_[1] for s_ in used]))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -620,10 +642,11 @@ def main():
     if not a.no_mbfs and info_sym(dg):
         # batched multi-source BFS against plain single-source BFS on the same resident
         # graph: 128-root launches over 1,024 seeded roots
-        r = mbfs_bench(a, prep, opts, dg)
+        r = mbfs_bench(a, prep, opts, dg, algos=(0, 1))
         mb = {"gteps": r["value"], "plain_gteps": r["plain_gteps"], "speedup_vs_plain": r["speedup_vs_plain"],
               "roots": r["config"]["roots"], "batch": r["config"]["batch"], "batch_ms": r["batch_ms"],
-              "plain_ms_per_root": r["plain_ms_per_root"], "algo": "bit-parallel"}
+              "plain_ms_per_root": r["plain_ms_per_root"], "algo": "bit-parallel",
+              "i8_mfma": r["by_algo"]["1"]}
     info = dg.info()
     cfg = {"workload": w.name}
     if w.kind == "rmat":

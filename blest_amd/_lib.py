@@ -41,7 +41,8 @@ class BvssInfo(C.Structure):
 class DeviceOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("alpha", C.c_double), ("beta", C.c_double), ("direction", C.c_int), ("keep_csr", C.c_int),
-                ("timeout_ms", C.c_uint32), ("use_graphs", C.c_int)]
+                ("timeout_ms", C.c_uint32), ("use_graphs", C.c_int), ("td_quads", C.c_uint32),
+                ("bu_group", C.c_uint32)]
 
 
 class DeviceGraphInfo(C.Structure):
@@ -141,6 +142,7 @@ PROTOTYPES = {
     "blest_mbfs": (C.c_int, [P, U32P, C.c_uint32, I32P, C.POINTER(MbfsOpts), C.POINTER(BfsStats)]),
     "blest_cc": (C.c_int, [P, C.c_int, U32P, C.POINTER(CcStats)]),
     "blest_device_graph_set_thresholds": (C.c_int, [P, C.c_double, C.c_double, C.c_int]),
+    "blest_device_graph_set_tiles": (C.c_int, [P, C.c_uint32, C.c_uint32]),
     "blest_autotune_bfs": (C.c_int, [P, P, U32P, U32P, C.c_uint32, C.POINTER(DeviceOpts), C.POINTER(DeviceOpts),
                                      DP, U32P]),
     "blest_host_alloc": (C.c_int, [C.c_size_t, PP]),

@@ -1,7 +1,10 @@
 // Launch-shape and direction-threshold autotuner for single-source BFS, built on the
 // C ABI.  Coordinate descent: segments per CU (a device-graph layout choice, so each
-// candidate is a fresh upload), then alpha, then beta (set in place).  Each candidate
-// is timed as the mean HIP-event time over the given roots after one warm-up pass;
+// candidate is a fresh upload), then the top-down slice tile (quads per lane per step)
+// and the bottom-up staging depth (words probed together), then alpha, then beta (set
+// in place).  Each candidate is timed as the mean HIP-event time over the given roots
+// after one warm-up pass, and replaces the best so far only when it is more than 1%
+// faster (candidates within timing noise keep the earlier, default-side choice);
 // results do not depend on the choice (levels are unique), only the time does.
 #include <cstdint>
 #include <limits>
@@ -55,6 +58,7 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
         if (base) o = *base;
         if (!g) o.keep_csr = 0;
         uint32_t runs = 0;
+        constexpr double kMargin = 0.99;  // a new best must be > 1% faster
         double bestMs = std::numeric_limits<double>::infinity();
         blest_device_opts bestOpts = o;
         Handles keep;  // graph of the best layout so far
@@ -66,12 +70,27 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
             check(blest_query_create(h->dg, 1, &h->q));
             const double ms = time_roots(h->q, roots, nroots);
             ++runs;
-            if (ms < bestMs) {
+            if (ms < bestMs * kMargin) {
                 bestMs = ms;
                 bestOpts = c;
                 std::swap(keep.dg, h->dg);
                 std::swap(keep.q, h->q);
             }
+        }
+        if (keep.dg) {
+            for (uint32_t tq : {2u, 1u, 4u})
+                for (uint32_t bg : {8u, 4u}) {
+                    if (tq == bestOpts.td_quads && bg == bestOpts.bu_group) continue;
+                    check(blest_device_graph_set_tiles(keep.dg, tq, bg));
+                    const double ms = time_roots(keep.q, roots, nroots);
+                    ++runs;
+                    if (ms < bestMs * kMargin) {
+                        bestMs = ms;
+                        bestOpts.td_quads = tq;
+                        bestOpts.bu_group = bg;
+                    }
+                }
+            check(blest_device_graph_set_tiles(keep.dg, bestOpts.td_quads, bestOpts.bu_group));
         }
         if (bestOpts.direction == 0 && keep.dg) {
             static const double alphas[] = {4, 8, 15, 30, 60, 120, 250, 500, 1000};
@@ -81,7 +100,7 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
                 check(blest_device_graph_set_thresholds(keep.dg, a, bestOpts.beta, 0));
                 const double ms = time_roots(keep.q, roots, nroots);
                 ++runs;
-                if (ms < bestMs) {
+                if (ms < bestMs * kMargin) {
                     bestMs = ms;
                     bestOpts.alpha = a;
                 }
@@ -91,7 +110,7 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
                 check(blest_device_graph_set_thresholds(keep.dg, bestOpts.alpha, be, 0));
                 const double ms = time_roots(keep.q, roots, nroots);
                 ++runs;
-                if (ms < bestMs) {
+                if (ms < bestMs * kMargin) {
                     bestMs = ms;
                     bestOpts.beta = be;
                 }

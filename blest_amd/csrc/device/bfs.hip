@@ -121,6 +121,11 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
     }
 }
 
+// Two launch-time knobs, both chosen by the autotuner: QPL = quads each lane streams per
+// top-down step (the slice tile a wave holds in flight: 64 x QPL quads = 256 x QPL
+// slices), BUG = bitmap words a bottom-up wave stages per group (vertices whose first
+// in-neighbour is probed together, BUG per lane).
+template <int QPL, int BUG>
 __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs];
@@ -160,12 +165,12 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             const unsigned incl = wave_inclusive_scan(qc);
             const unsigned excl = incl - qc;
             const unsigned tq = __shfl(incl, kWave - 1, kWave);
-            // two quads per lane per step, so two row/mask loads are in flight
-            for (unsigned t0 = 0; t0 < tq; t0 += 2 * kWave) {
-                uint4 r4[2];
-                uint32_t m[2];
+            // QPL quads per lane per step, so QPL row/mask loads are in flight
+            for (unsigned t0 = 0; t0 < tq; t0 += QPL * kWave) {
+                uint4 r4[QPL];
+                uint32_t m[QPL];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < QPL; ++h) {
                     const unsigned t = t0 + h * kWave + lane;
                     const unsigned owner = wave_owner(excl, min(t, tq - 1));
                     const uint32_t oq0 = __shfl(q0, owner, kWave);
@@ -180,7 +185,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                     }
                 }
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
+                for (int h = 0; h < QPL; ++h) {
                     expand_slice(r4[h].x, m[h] & 0xFF, a.visited, a.nextB);
                     expand_slice(r4[h].y, (m[h] >> 8) & 0xFF, a.visited, a.nextB);
                     expand_slice(r4[h].z, (m[h] >> 16) & 0xFF, a.visited, a.nextB);
@@ -235,7 +240,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     // for the whole level.  (A fully wave-cooperative scan, lanes dealt out over the
     // word's open vertices every step, measured slower: its per-step shuffles cost as
     // much as a memory round trip.)
-    constexpr int kBuGroup = 8, kBuBatch = 4;
+    constexpr int kBuGroup = BUG, kBuBatch = 4;
     constexpr uint32_t kBuBudget = 32;
     constexpr unsigned kHeavyCap = 1024;
     __shared__ uint32_t heavyU[kHeavyCap], heavyP[kHeavyCap];
@@ -504,6 +509,17 @@ __global__ void unpermute_levels_kernel(int32_t* out, const int32_t* in, const u
 
 }  // namespace dev
 
+// The expand kernel instantiation for the graph's tile knobs (validated at creation).
+using ExpandKernel = void (*)(GraphView, dev::BfsArgs, unsigned);
+static ExpandKernel expand_kernel_for(const DeviceGraph& g) {
+    const bool g4 = g.opts.bu_group == 4;
+    switch (g.opts.td_quads) {
+        case 1: return g4 ? dev::bfs_expand_kernel<1, 4> : dev::bfs_expand_kernel<1, 8>;
+        case 4: return g4 ? dev::bfs_expand_kernel<4, 4> : dev::bfs_expand_kernel<4, 8>;
+        default: return g4 ? dev::bfs_expand_kernel<2, 4> : dev::bfs_expand_kernel<2, 8>;
+    }
+}
+
 void bfs_launch(Query& q, uint32_t src, bool trace) {
     DeviceGraph& g = *q.g;
     dev::BfsArgs a;
@@ -528,6 +544,7 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     ar.levelFromCtrl = 1;
     const GraphView gv = g.view();
     const dim3 grid = level_grid(g), block(dev::kProducerBlock);
+    const ExpandKernel expand = expand_kernel_for(g);
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
     hipLaunchKernelGGL(dev::bfs_init_kernel, grid, block, 0, q.stream, gv, a);
     if (trace) {
@@ -544,7 +561,7 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
             q,
             [&](unsigned L) {
                 mark();
-                hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, a, L);
+                hipLaunchKernelGGL(expand, grid, block, 0, q.stream, gv, a, L);
                 mark();
                 hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, a, L);
                 mark();
@@ -559,16 +576,16 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     run_levels(
         q,
         [&](unsigned L) {
-            hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(expand, grid, block, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, a, L);
         },
         [&] { hipLaunchKernelGGL(dev::bfs_final_kernel, grid, block, 0, q.stream, gv, a); },
         &q.graphs[kGraphBfs],
         [&](unsigned k) {
-            hipLaunchKernelGGL(dev::bfs_expand_kernel, grid, block, 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL(expand, grid, block, 0, q.stream, gv, ar, k);
             hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, ar, k);
         },
-        launch_key(a.alpha, a.beta, a.direction, 0));
+        launch_key(a.alpha, a.beta, a.direction, g.opts.td_quads | (g.opts.bu_group << 8)));
     BLEST_HIP(hipGetLastError());
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     q.lastSource = src;

@@ -143,6 +143,8 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->keep_csr = 1;
     o->timeout_ms = 20000;
     o->use_graphs = 0;
+    o->td_quads = 2;
+    o->bu_group = 8;
 }
 
 }  // extern "C"
@@ -167,6 +169,11 @@ static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* g
         if (opts.keep_csr && !gh)
             throw Error(BLEST_ERR_INVALID_ARGUMENT, ghIsCsr ? "keep_csr needs the CSR" : "keep_csr needs the CSC");
         if (opts.use_graphs < 0 || opts.use_graphs > 2) throw Error(BLEST_ERR_INVALID_ARGUMENT, "use_graphs must be 0, 1 or 2");
+        if (opts.td_quads == 0) opts.td_quads = 2;
+        if (opts.bu_group == 0) opts.bu_group = 8;
+        if (opts.td_quads != 1 && opts.td_quads != 2 && opts.td_quads != 4)
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, "td_quads must be 1, 2 or 4");
+        if (opts.bu_group != 4 && opts.bu_group != 8) throw Error(BLEST_ERR_INVALID_ARGUMENT, "bu_group must be 4 or 8");
         auto* h = new blest_device_graph();
         DeviceGraph& g = h->g;
         try {
@@ -341,6 +348,17 @@ blest_status blest_device_graph_set_thresholds(blest_device_graph* h, double alp
         h->g.opts.alpha = alpha;
         h->g.opts.beta = beta;
         h->g.opts.direction = direction;
+    });
+}
+
+blest_status blest_device_graph_set_tiles(blest_device_graph* h, uint32_t td_quads, uint32_t bu_group) {
+    return guarded([&] {
+        if (!h) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null argument");
+        if (td_quads != 1 && td_quads != 2 && td_quads != 4)
+            throw Error(BLEST_ERR_INVALID_ARGUMENT, "td_quads must be 1, 2 or 4");
+        if (bu_group != 4 && bu_group != 8) throw Error(BLEST_ERR_INVALID_ARGUMENT, "bu_group must be 4 or 8");
+        h->g.opts.td_quads = td_quads;
+        h->g.opts.bu_group = bu_group;
     });
 }
 

@@ -199,7 +199,113 @@ constexpr unsigned mbfs_expand_block() {
     return NW == 1 ? kProducerBlock : kProducerBlock / 2;
 }
 
+// Top-down level as an i8 matrix product on the MFMA units (algo 1).  For a vset of
+// rset r (columns 8r..8r+7) the contributions of its slices form the boolean product
+// C[slice][root] = OR_b A[slice][b] AND F[8r+b][root], A = the slices' 8-bit masks,
+// F = the 8 columns' frontier words.  Computed exactly as an integer product with 0/1
+// i8 entries and a > 0 test: v_mfma_i32_32x32x16_i8 multiplies 32 slices (M) by 32
+// roots (N) over K = 16, of which the rset's 8 columns fill K 0..7 and K 8..15 stay
+// zero.  A wave takes one vset at a time (every slice of a tile shares the B operand of
+// its rset) and 64 slices per step: tile 0 = slices s0..s0+31, tile 1 = s0+32..s0+63,
+// each 2 x NW products (32 roots apiece).  Operand layouts (CDNA3/4, 32x32x16 i8):
+// lane l holds A[l % 32][8 (l / 32) + 0..7] and B[8 (l / 32) + 0..7][l % 32] as 8 bytes;
+// result register j of lane l is C[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].  A ballot
+// of (C > 0) per result register therefore yields two rows' 32-root masks, and lane m
+// (tile 0) or m + 32 (tile 1) collects row m's bits.  Then, as in the VALU path, a slice
+// whose row is not saturated ORs the roots that have not reached the row into its
+// `next` word.
+typedef int mfma_i32x16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ long mfma_bytes_of_bits(uint32_t bits8) {
+    unsigned long long v = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v |= (unsigned long long)((bits8 >> b) & 1u) << (8 * b);
+    return (long)v;
+}
+
 template <int NW>
+__device__ __forceinline__ void mbfs_td_mfma(const GraphView& g, const MbfsArgs& a, const uint32_t* lin,
+                                             const unsigned* pre, unsigned total,
+                                             const unsigned long long* fin) {
+    const unsigned lane = lane_id();
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
+    const uint32_t* rowIds = reinterpret_cast<const uint32_t*>(g.rows);
+    const uint8_t* maskBytes = reinterpret_cast<const uint8_t*>(g.masks);
+    constexpr int kGroups = 2 * NW;  // 32-root groups
+    // this lane's result row: tile (lane / 32), row m = lane % 32 = 8 (j / 4) + 4 h + j % 4
+    const unsigned m = lane & 31u;
+    const unsigned jm = 4 * (m >> 3) + (m & 3u), hm = (m >> 2) & 1u;
+    for (unsigned e = wave; e < total; e += nwaves) {
+        const uint32_t vs = __builtin_amdgcn_readfirstlane(list_entry(lin, a.sv, pre, e));
+        const uint32_t q0 = g.unitQuad[vs];
+        const uint32_t nsl = (g.unitQuad[vs + 1] - q0) * 4;
+        const uint32_t r = g.unitRset[vs];
+        // the rset's frontier block (wave-uniform) and the B operands built from it
+        unsigned long long F[8][NW];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int i = 0; i < NW; ++i) F[b][i] = (8 * r + b < g.n) ? fin[size_t(8 * r + b) * NW + i] : 0ull;
+        long Bop[kGroups];
+#pragma unroll
+        for (int t = 0; t < kGroups; ++t) {
+            uint32_t bits = 0;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) bits |= uint32_t((F[b][t / 2] >> (32 * (t & 1) + (lane & 31u))) & 1ull) << b;
+            Bop[t] = lane < 32 ? mfma_bytes_of_bits(bits) : 0l;
+        }
+        for (uint32_t s0 = 0; s0 < nsl; s0 += 64) {
+            const uint32_t sl = s0 + lane;
+            uint32_t row = 0, mk = 0;
+            if (sl < nsl) {
+                const size_t slot = size_t(q0) * 4 + sl;
+                row = rowIds[slot];
+                mk = maskBytes[slot];
+            }
+            // A operands: lanes 0..31 hold tile 0's slice masks (their own), tile 1's come
+            // from lanes 32..63; lanes 32..63 hold the zero K half
+            const uint32_t mkHi = __shfl(mk, (lane & 31u) + 32u, kWave);
+            const long A0 = lane < 32 ? mfma_bytes_of_bits(mk) : 0l;
+            const long A1 = lane < 32 ? mfma_bytes_of_bits(mkHi) : 0l;
+            Bits<NW> acc;
+            acc.zero();
+#pragma unroll
+            for (int tile = 0; tile < 2; ++tile) {
+#pragma unroll
+                for (int t = 0; t < kGroups; ++t) {
+                    mfma_i32x16_t c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    c = __builtin_amdgcn_mfma_i32_32x32x16_i8(tile ? A1 : A0, Bop[t], c, 0, 0, 0);
+                    unsigned long long mine = 0;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const unsigned long long bal = __ballot(c[j] > 0);
+                        mine = (unsigned(j) == jm) ? bal : mine;
+                    }
+                    const unsigned long long part = hm ? (mine >> 32) : (mine & 0xFFFFFFFFull);
+                    if ((lane >> 5) == unsigned(tile)) acc.w[t / 2] |= part << (32 * (t & 1));
+                }
+            }
+            bool act = mk != 0 && acc.any();
+            const unsigned long long fullw = act ? a.full[row >> 6] : 0ull;
+            act = act && !((fullw >> (row & 63)) & 1ull);
+            if (act) {
+                const unsigned long long* rec = vis_at<NW>(a, row);
+                unsigned long long seen[NW];
+#pragma unroll
+                for (int i = 0; i < NW; ++i) seen[i] = rec[i] | rec[NW + i];
+                unsigned long long* nr = next_at<NW>(a, row);
+#pragma unroll
+                for (int i = 0; i < NW; ++i) {
+                    const unsigned long long nv = acc.w[i] & ~seen[i];
+                    if (nv) atomicOr(nr + i, nv);
+                }
+            }
+        }
+    }
+}
+
+template <int NW, int ALGO>
 __global__ void __launch_bounds__(kProducerBlock) __attribute__((amdgpu_waves_per_eu(NW == 1 ? 8 : 5)))
 mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
@@ -222,6 +328,10 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     unsigned long long* fout = mfront<NW>(a, g.n, L);
     const bool writeLevels = L > a.histLevels;
     const unsigned sOut = (L + 1) % 3;
+    if (!bottomUp && ALGO == 1) {
+        mbfs_td_mfma<NW>(g, a, (L & 1) ? a.list1 : a.list0, pre, total, fin);
+        return;
+    }
     if (!bottomUp) {
         const uint32_t* lin = (L & 1) ? a.list1 : a.list0;
 
@@ -629,7 +739,7 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     q.mbfsWords = nw;
 }
 
-template <int NW>
+template <int NW, int ALGO>
 static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     DeviceGraph& g = *q.g;
     const GraphView gv = g.view();
@@ -642,7 +752,7 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
     run_levels(
         q,
         [&](unsigned L) {
-            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, eblock, 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL((dev::mbfs_expand_kernel<NW, ALGO>), grid, eblock, 0, q.stream, gv, a, L);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, a, L);
         },
         [&] {
@@ -651,10 +761,10 @@ static void mbfs_enqueue_levels(Query& q, const dev::MbfsArgs& a) {
         },
         &q.graphs[NW == 1 ? kGraphMbfs1 : kGraphMbfs2],
         [&](unsigned k) {
-            hipLaunchKernelGGL(dev::mbfs_expand_kernel<NW>, grid, eblock, 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL((dev::mbfs_expand_kernel<NW, ALGO>), grid, eblock, 0, q.stream, gv, ar, k);
             hipLaunchKernelGGL(dev::mbfs_update_kernel<NW>, grid, block, 0, q.stream, gv, ar, k);
         },
-        launch_key(a.alpha, a.beta, a.direction, a.k));
+        launch_key(a.alpha, a.beta, a.direction, a.k | (uint32_t(ALGO) << 16)));
     BLEST_HIP(hipGetLastError());
 }
 
@@ -664,7 +774,8 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     DeviceGraph& g = *q.g;
     if (k == 0 || k > 128) throw Error(BLEST_ERR_INVALID_ARGUMENT, "multi-source batches hold 1..128 roots");
     if (k > q.maxSources) throw Error(BLEST_ERR_INVALID_ARGUMENT, "more roots than the query's max_sources");
-    if (o && o->algo != 0) throw Error(BLEST_ERR_UNSUPPORTED, "only the bit-parallel multi-source kernel is built");
+    const int algo = o ? o->algo : 0;
+    if (algo != 0 && algo != 1) throw Error(BLEST_ERR_INVALID_ARGUMENT, "algo must be 0 (bit-parallel) or 1 (i8 MFMA)");
     const uint32_t nw = (k > 64 || (o && o->batch > 64)) ? 2 : 1;
     mbfs_ensure_state(q, nw);
     std::vector<uint32_t> mapped(k);
@@ -694,8 +805,13 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.beta = g.opts.beta;
     a.levelFromCtrl = 0;
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
-    if (nw == 1) mbfs_enqueue_levels<1>(q, a);
-    else mbfs_enqueue_levels<2>(q, a);
+    if (nw == 1) {
+        if (algo == 1) mbfs_enqueue_levels<1, 1>(q, a);
+        else mbfs_enqueue_levels<1, 0>(q, a);
+    } else {
+        if (algo == 1) mbfs_enqueue_levels<2, 1>(q, a);
+        else mbfs_enqueue_levels<2, 0>(q, a);
+    }
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     BLEST_HIP(hipStreamSynchronize(q.stream));
     dev::Ctrl c;

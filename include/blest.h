@@ -195,6 +195,10 @@ typedef struct blest_device_opts {
     int use_graphs;          /* level chunks replayed from a captured HIP graph: 0 = once a
                                 query's previous traversal was >= 64 levels deep (default),
                                 1 = always, 2 = never.  Queries of a request server never replay. */
+    uint32_t td_quads;       /* slice tile of a top-down wave step: quads (4 slices) each lane
+                                streams per step, 1, 2 (default) or 4 */
+    uint32_t bu_group;       /* bottom-up staging depth: bitmap words a wave probes together, 4
+                                or 8 (default) */
 } blest_device_opts;
 void blest_device_opts_default(blest_device_opts* o);
 
@@ -227,10 +231,17 @@ blest_status blest_device_graph_info_get(const blest_device_graph* dg, blest_dev
  * traversal on the graph. */
 blest_status blest_device_graph_set_thresholds(blest_device_graph* dg, double alpha, double beta, int direction);
 
-/* Autotuner (no reference counterpart; the reference hard-codes its launch shape):
- * coordinate descent over blocks_per_cu (1, 2, 4, 8), then alpha, then beta, each
+/* Change the tile knobs (td_quads, bu_group) of a device graph; same rules as
+ * blest_device_graph_set_thresholds. */
+blest_status blest_device_graph_set_tiles(blest_device_graph* dg, uint32_t td_quads, uint32_t bu_group);
+
+/* Autotuner (no reference counterpart; the reference hard-codes its launch shape,
+ * BFS/BVSSBFSKernel.cuh:1311-1358): coordinate descent over blocks_per_cu (1, 2, 4, 8),
+ * then td_quads (1, 2, 4), bu_group (4, 8), alpha, then beta, each
  * candidate timed as the mean HIP-event time of single-source BFS from `roots`
- * (original IDs) after one warm-up pass.  base (nullable) supplies the other fields;
+ * (original IDs) after one warm-up pass; a candidate replaces the best so far only when
+ * it is more than 1% faster, so ties between candidates within timing noise keep the
+ * earlier (default-side) choice.  base (nullable) supplies the other fields;
  * best receives the fastest opts, best_ms its mean time, trials the candidates run. */
 blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc* g, const uint32_t* perm_old_to_new,
                                 const uint32_t* roots, uint32_t nroots, const blest_device_opts* base,

@@ -20,7 +20,9 @@ def test_autotune_then_parity(blest, oracle):
     assert best["blocks_per_cu"] in (1, 2, 4, 8)
     assert best["alpha"] in (4, 8, 15, 30, 60, 120, 250, 500, 1000)
     assert best["beta"] in (6, 12, 18, 24, 48, 96, 200, 400, 800)
-    dg = blest.DeviceGraph(b, g, inv, **{k: best[k] for k in ("blocks_per_cu", "alpha", "beta")})
+    assert best["td_quads"] in (1, 2, 4) and best["bu_group"] in (4, 8)
+    dg = blest.DeviceGraph(b, g, inv, **{k: best[k] for k in ("blocks_per_cu", "td_quads", "bu_group", "alpha",
+                                                                "beta")})
     k = blest.BVSSBFSKernel(dg)
     for r in roots_orig:
         np.testing.assert_array_equal(k.run(int(r)).levels, oracle.bfs(g.n, cp, rows, int(r)))
@@ -72,3 +74,38 @@ def test_bfs_trace_matches_plain_run(blest, oracle):
         for i in range(cnt.value):
             assert tr[i].level == i + 1 and tr[i].expand_ms >= 0 and tr[i].update_ms >= 0
             assert tr[i].bottom_up == ((st.bottom_up_mask >> i) & 1)
+
+
+@pytest.mark.parametrize("td_quads", [1, 2, 4])
+@pytest.mark.parametrize("bu_group", [4, 8])
+@pytest.mark.parametrize("direction", [0, 2])
+def test_tile_knobs(blest, oracle, td_quads, bu_group, direction):
+    """Every top-down slice tile and bottom-up staging depth gives the oracle's levels,
+    at creation and when changed in place."""
+    print(f"case tiles td_quads={td_quads} bu_group={bu_group} direction={direction}", flush=True)
+    g = blest.CSC.rmat(13, 16, seed=14)
+    cp, rows = g.arrays()
+    inv = g.reorder(blest.Reorder.JACCARD, window=2048)
+    b = blest.BVSS(8, 4).construct_from_csc(g)
+    dg = blest.DeviceGraph(b, g, inv, td_quads=td_quads, bu_group=bu_group, direction=direction)
+    k = blest.BVSSBFSKernel(dg)
+    roots = [int(r) for r in np.argsort(inv)[g.pick_roots(3, seed=3)]]
+    for r in roots:
+        np.testing.assert_array_equal(k.run(r).levels, oracle.bfs(g.n, cp, rows, r))
+    lib = blest._lib.load()
+    blest._lib.check(lib.blest_device_graph_set_tiles(dg._h, 4 if td_quads != 4 else 1, 12 - bu_group))
+    for r in roots:
+        np.testing.assert_array_equal(k.run(r).levels, oracle.bfs(g.n, cp, rows, r))
+
+
+def test_tile_knobs_rejected(blest):
+    g = blest.CSC.rmat(10, 8, seed=8)
+    b = blest.BVSS(8, 4).construct_from_csc(g)
+    with pytest.raises(blest.BlestError):
+        blest.DeviceGraph(b, g, None, td_quads=3)
+    with pytest.raises(blest.BlestError):
+        blest.DeviceGraph(b, g, None, bu_group=16)
+    dg = blest.DeviceGraph(b, g, None)
+    lib = blest._lib.load()
+    assert lib.blest_device_graph_set_tiles(dg._h, 8, 8) == 1
+    assert lib.blest_device_graph_set_tiles(dg._h, 2, 5) == 1

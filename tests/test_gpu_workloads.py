@@ -82,6 +82,11 @@ def test_mbfs_128_roots_rmat22(blest, oracle):
     for j, r in enumerate(roots):
         np.testing.assert_array_equal(res.level_arrays[j], single.run(int(r)).levels,
                                       err_msg=f"row {j} root {r}: multi-source vs single-source")
+    # the same launch through the i8 MFMA product: every row equal
+    mf = blest.BVSSMBFSKernel(p.bvss, p.csc, p.inv_perm, batch=128, algo=1).run(roots)
+    for j in range(128):
+        np.testing.assert_array_equal(mf.level_arrays[j], res.level_arrays[j], err_msg=f"row {j}: MFMA vs VALU")
+    del mf
     rows_checked = [0, 9, 18, 31, 40, 52, 63, 64, 71, 80, 95, 100, 111, 119, 126, 127]
     for j, (r, want) in zip(rows_checked, oracle.bfs_many(n, p.orig_col_ptrs, p.orig_rows,
                                                           [int(roots[j]) for j in rows_checked])):
