@@ -134,6 +134,7 @@ struct Query {
     unsigned long long *mVis = nullptr, *mFront0 = nullptr, *mFront1 = nullptr, *mNext = nullptr;
     unsigned long long* mHist = nullptr;  // frontier words of levels 0..mHistLevels
     unsigned long long* mFull = nullptr;  // rows every root has reached (1 bit per vertex)
+    unsigned long long* mAny = nullptr;   // 2 x W: vertices in the frontier of some root (1 bit), by level parity
     uint32_t mHistLevels = 0;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries

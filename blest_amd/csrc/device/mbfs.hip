@@ -30,6 +30,8 @@ struct MbfsArgs {
     unsigned long long* hist;    // frontier words of levels 0..histLevels (n * NW each)
     uint32_t histLevels;
     unsigned long long* full;    // bit v: every root has reached v (top-down skips its record)
+    unsigned long long* any;     // 2 x W: bit v of half L & 1: v is in the frontier of level L for some root
+    int pull;                    // symmetric graph: bottom-up levels pull over the BVSS slices
     uint32_t* list0;
     uint32_t* list1;
     const uint32_t* srcs;
@@ -152,7 +154,11 @@ template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_init_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     unsigned long long* f0 = a.hist;  // level 0
-    for (uint32_t w = tid; w < g.W; w += nthreads) a.full[w] = 0ull;
+    for (uint32_t w = tid; w < g.W; w += nthreads) {
+        a.full[w] = 0ull;
+        a.any[w] = 0ull;
+        a.any[g.W + w] = 0ull;
+    }
     for (size_t i = tid; i < size_t(g.n) * NW; i += nthreads) {
         f0[i] = 0ull;
         a.vn[2 * i] = 0ull;
@@ -177,6 +183,7 @@ __global__ void mbfs_seed_kernel(GraphView g, MbfsArgs a) {
     const unsigned long long bit = 1ull << (j & 63);
     atomicOr(vis_at<NW>(a, v) + j / 64, bit);
     atomicOr(&a.hist[size_t(v) * NW + j / 64], bit);
+    atomicOr(&a.any[v >> 6], 1ull << (v & 63));  // level 0: parity half 0
     a.levels[size_t(j) * g.n + v] = 0;
     const uint32_t seg = (v >> 6) / a.sv.wps;
     atomicAdd(&a.sc.news[a.sv.segs + seg], 1u);
@@ -197,6 +204,84 @@ __global__ void __launch_bounds__(kProducerBlock) mbfs_first_list_kernel(GraphVi
 template <int NW>
 constexpr unsigned mbfs_expand_block() {
     return NW == 1 ? kProducerBlock : kProducerBlock / 2;
+}
+
+// Bottom-up level of a symmetric graph as a pull over the same BVSS slices.  For a
+// symmetric graph a slice (row i, mask m) of rset r is also the set of edges
+// i -> 8r + b (b in m), so vertex 8r + b gains OR_{slices with bit b} F[i], F = the
+// previous level's frontier words.  A vset is taken by 16 lanes (two quads each per
+// step, a vset holding at most 32 quads); a row's frontier words are loaded only when
+// its frontier-any bit is set (the bitmap sits in L2), the 8 accumulators are OR-reduced
+// over the 16 lanes, and lane b of the group ORs vertex 8r + b's new roots (those not
+// yet seen there) into its `next` word: a handful of atomics per vset, none per slice,
+// where the push expansion did a random record read and an atomic per slice.  Rsets
+// whose 8 vertices every root has reached are skipped without reading their slices.
+// The update kernel folds `next` exactly as after a push level.
+template <int NW>
+__device__ __forceinline__ void mbfs_pull(const GraphView& g, const MbfsArgs& a, const unsigned long long* fin,
+                                          const unsigned long long* anyIn) {
+    const unsigned lane = lane_id(), sub = lane & 15u, grp = lane >> 4;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const unsigned nwaves = (gridDim.x * blockDim.x) / kWave;
+    for (uint32_t v0 = wave * 4; v0 < g.V; v0 += nwaves * 4) {
+        const uint32_t vs = v0 + grp;
+        bool ok = vs < g.V;
+        uint32_t r = 0, q0 = 0, qc = 0;
+        if (ok) {
+            r = g.unitRset[vs];
+            q0 = g.unitQuad[vs];
+            qc = g.unitQuad[vs + 1] - q0;
+            ok = ((a.full[r >> 3] >> (8 * (r & 7))) & 0xFFull) != 0xFFull;
+        }
+        Bits<NW> acc[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[b].zero();
+        for (uint32_t qq = sub; ok && qq < qc; qq += 16) {
+            const uint4 r4 = stream_load_quad(g.rows, q0 + qq);
+            const uint32_t m = stream_load_u32(g.masks, q0 + qq);
+            const uint32_t rowv[4] = {r4.x, r4.y, r4.z, r4.w};
+            bool hit[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) hit[s] = ((m >> (8 * s)) & 0xFFu) != 0 && test_bit(anyIn, rowv[s]);
+            Bits<NW> Fi[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                Fi[s].zero();
+                if (hit[s]) Fi[s].load(fin, rowv[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t ms = hit[s] ? (m >> (8 * s)) & 0xFFu : 0u;
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if ((ms >> b) & 1u)
+#pragma unroll
+                        for (int i = 0; i < NW; ++i) acc[b].w[i] |= Fi[s].w[i];
+            }
+        }
+        // OR over the group's 16 lanes (xor distances below 16 stay inside the group)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+#pragma unroll
+                for (int d = 8; d > 0; d >>= 1) acc[b].w[i] |= __shfl_xor(acc[b].w[i], d, kWave);
+        Bits<NW> mine;
+        mine.zero();
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            if (sub == unsigned(b)) mine = acc[b];
+        const uint32_t v = 8 * r + sub;
+        if (ok && sub < 8 && v < g.n && mine.any()) {
+            const unsigned long long* rec = vis_at<NW>(a, v);
+            unsigned long long* nr = next_at<NW>(a, v);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                const unsigned long long nv = mine.w[i] & ~(rec[i] | rec[NW + i]);
+                if (nv) atomicOr(nr + i, nv);
+            }
+        }
+    }
 }
 
 // Top-down level as an i8 matrix product on the MFMA units (algo 1).  For a vset of
@@ -315,7 +400,7 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const uint32_t segs = a.sv.segs;
     DirectionParams dp;
     dp.direction = a.direction;
-    dp.canBottomUp = g.csrPtr != nullptr;
+    dp.canBottomUp = g.csrPtr != nullptr || a.pull;
     dp.alpha = a.alpha;
     dp.beta = a.beta;
     dp.totalEdges = (unsigned long long)a.k * g.nnz;
@@ -330,6 +415,10 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
     const unsigned sOut = (L + 1) % 3;
     if (!bottomUp && ALGO == 1) {
         mbfs_td_mfma<NW>(g, a, (L & 1) ? a.list1 : a.list0, pre, total, fin);
+        return;
+    }
+    if (bottomUp && a.pull) {
+        mbfs_pull<NW>(g, a, fin, a.any + size_t((L - 1) & 1) * g.W);
         return;
     }
     if (!bottomUp) {
@@ -540,7 +629,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->histTop = L;  // read only after this kernel
-    if (bottomUp) {
+    if (bottomUp && !a.pull) {
         const unsigned cnt = segment_list<NW>(g, a.sv, fout, lout, l);
         if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = cnt;
         return;
@@ -598,7 +687,11 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
                 nw.store(fout, v);
             }
             const unsigned long long fullBits = __ballot(nowFull);
-            if (fullBits && lane_id() == 0 && w < last) a.full[w] |= fullBits;  // this wave owns word w here
+            const unsigned long long anyBits = __ballot(ok[j] && nx[j].any());
+            if (lane_id() == 0 && w < last) {  // this wave owns word w here
+                if (fullBits) a.full[w] |= fullBits;
+                a.any[size_t(L & 1) * g.W + w] = anyBits;
+            }
         }
     }
     __syncthreads();  // the frontier words above are read by other waves below
@@ -719,7 +812,7 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     DeviceGraph& g = *q.g;
     if (q.mbfsWords >= nw) return;
     for (void* p : {static_cast<void*>(q.mVis), static_cast<void*>(q.mFront0), static_cast<void*>(q.mFront1),
-                    static_cast<void*>(q.mHist), static_cast<void*>(q.mFull)})
+                    static_cast<void*>(q.mHist), static_cast<void*>(q.mFull), static_cast<void*>(q.mAny)})
         q.release(p);
     // a new layout invalidates every captured multi-source level chunk
     for (int k : {kGraphMbfs1, kGraphMbfs2})
@@ -734,6 +827,7 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     q.mHistLevels = mbfs_history_levels(g.n, nw);
     q.mHist = q.alloc<unsigned long long>(words * (q.mHistLevels + 1));
     q.mFull = q.alloc<unsigned long long>(g.W);
+    q.mAny = q.alloc<unsigned long long>(2 * size_t(g.W));
     if (!q.srcs) q.srcs = q.alloc<uint32_t>(128);
     if (!q.rootStats) q.rootStats = q.alloc<dev::RootStats>(128);
     q.mbfsWords = nw;
@@ -789,6 +883,8 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.hist = q.mHist;
     a.histLevels = q.mHistLevels;
     a.full = q.mFull;
+    a.any = q.mAny;
+    a.pull = g.symmetric ? 1 : 0;
     a.list0 = q.list0;
     a.list1 = q.list1;
     a.srcs = q.srcs;
@@ -796,11 +892,16 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.ctrl = q.ctrl;
     a.sv = g.seg_view();
     a.sc = q.seg_counters();
-    // Direction: top-down unless bottom-up is asked for explicitly (direction 2).  With
-    // 64-128 roots a bottom-up scan must find every root still missing at a vertex, so
-    // it seldom stops early; on R-MAT scale 22 (128 roots) the switching schedule took
-    // 21 ms per batch and top-down only 9.6 ms.
-    a.direction = (g.hasCsr && g.opts.direction == 2) ? 2 : 1;
+    // Direction.  A symmetric graph pulls over the BVSS slices on bottom-up levels
+    // (mbfs_pull), and switches by the graph's thresholds (direction 0) unless one
+    // direction is asked for.  Otherwise top-down unless bottom-up (the CSR scan) is
+    // asked for explicitly (direction 2): with 64-128 roots a per-vertex scan must find
+    // every root still missing at a vertex, so it seldom stops early; on R-MAT scale 22
+    // (128 roots) that switching schedule took 21 ms per batch and top-down only 9.6 ms.
+    if (g.symmetric)
+        a.direction = g.opts.direction;
+    else
+        a.direction = (g.hasCsr && g.opts.direction == 2) ? 2 : 1;
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
     a.levelFromCtrl = 0;
