@@ -445,7 +445,7 @@ blest_status blest_query_footprint(const blest_device_graph* dg, uint32_t max_so
         if (max_sources > 1) {
             const uint32_t nw = max_sources > 64 ? 2 : 1;
             b += (4ull + mbfs_history_levels(g.n, nw) + 1) * 8ull * g.n * nw + 4ull * 128 + 16ull * 128;
-            b += 2ull * 8ull * g.W;  // frontier-any bitmaps
+            b += 2ull * 8ull * g.W + g.W;  // frontier-any bitmaps, dirty word marks
         }
         b += 2ull * 4ull * g.n;  // CC labels
         *bytes = b;

@@ -135,6 +135,7 @@ struct Query {
     unsigned long long* mHist = nullptr;  // frontier words of levels 0..mHistLevels
     unsigned long long* mFull = nullptr;  // rows every root has reached (1 bit per vertex)
     unsigned long long* mAny = nullptr;   // 2 x W: vertices in the frontier of some root (1 bit), by level parity
+    uint8_t* mDirty = nullptr;            // W: words whose `next` words were set in the running level
     uint32_t mHistLevels = 0;
     uint32_t* srcs = nullptr;  // multi-source roots (reordered IDs), 128 entries
     void* rootStats = nullptr;  // per-root totals of a multi-source run, 128 entries

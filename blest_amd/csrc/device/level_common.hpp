@@ -97,7 +97,10 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
     if (p.direction == 0 && p.canBottomUp) {
         const double mu = double(p.totalEdges - min(cum, p.totalEdges));
         if (!bottomUp) {
-            if (double(mf) * p.alpha > mu) bottomUp = true;
+            // only while the frontier grows: a shrinking tail frontier whose edges exceed
+            // what is left unexplored (edges of unreachable vertices stay unexplored) would
+            // otherwise sweep every vertex again
+            if (double(mf) * p.alpha > mu && nf > in.prevNf) bottomUp = true;
         } else if (double(nf) * p.beta < p.totalItems && nf < in.prevNf) {
             bottomUp = false;
         }

@@ -32,6 +32,7 @@ struct MbfsArgs {
     unsigned long long* full;    // bit v: every root has reached v (top-down skips its record)
     unsigned long long* any;     // 2 x W: bit v of half L & 1: v is in the frontier of level L for some root
     int pull;                    // symmetric graph: bottom-up levels pull over the BVSS slices
+    uint8_t* dirty;              // W bytes: word w holds a vertex whose `next` word was set this level
     uint32_t* list0;
     uint32_t* list1;
     const uint32_t* srcs;
@@ -154,10 +155,17 @@ template <int NW>
 __global__ void __launch_bounds__(kProducerBlock) mbfs_init_kernel(GraphView g, MbfsArgs a) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     unsigned long long* f0 = a.hist;  // level 0
-    for (uint32_t w = tid; w < g.W; w += nthreads) {
-        a.full[w] = 0ull;
-        a.any[w] = 0ull;
-        a.any[g.W + w] = 0ull;
+    // vertices without edges can only ever be reached by their own root: they start
+    // saturated, so no slice or rset waits on them (the slots past n likewise)
+    for (uint32_t v = tid; v < g.W * 64; v += nthreads) {  // whole waves: 64 consecutive v
+        const bool isolated = v >= g.n || (g.csrPtr && g.csrPtr[v + 1] == g.csrPtr[v]);
+        const unsigned long long bits = __ballot(isolated);
+        if (lane_id() == 0) {
+            a.full[v >> 6] = bits;
+            a.any[v >> 6] = 0ull;
+            a.any[g.W + (v >> 6)] = 0ull;
+            a.dirty[v >> 6] = 0;
+        }
     }
     for (size_t i = tid; i < size_t(g.n) * NW; i += nthreads) {
         f0[i] = 0ull;
@@ -226,12 +234,13 @@ __device__ __forceinline__ void mbfs_pull(const GraphView& g, const MbfsArgs& a,
     for (uint32_t v0 = wave * 4; v0 < g.V; v0 += nwaves * 4) {
         const uint32_t vs = v0 + grp;
         bool ok = vs < g.V;
-        uint32_t r = 0, q0 = 0, qc = 0;
+        uint32_t r = 0, q0 = 0, qc = 0, open = 0;  // open: columns some root has not reached
         if (ok) {
             r = g.unitRset[vs];
             q0 = g.unitQuad[vs];
             qc = g.unitQuad[vs + 1] - q0;
-            ok = ((a.full[r >> 3] >> (8 * (r & 7))) & 0xFFull) != 0xFFull;
+            open = ~uint32_t(a.full[r >> 3] >> (8 * (r & 7))) & 0xFFu;
+            ok = open != 0;
         }
         Bits<NW> acc[8];
 #pragma unroll
@@ -242,7 +251,7 @@ __device__ __forceinline__ void mbfs_pull(const GraphView& g, const MbfsArgs& a,
             const uint32_t rowv[4] = {r4.x, r4.y, r4.z, r4.w};
             bool hit[4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) hit[s] = ((m >> (8 * s)) & 0xFFu) != 0 && test_bit(anyIn, rowv[s]);
+            for (int s = 0; s < 4; ++s) hit[s] = ((m >> (8 * s)) & open) != 0 && test_bit(anyIn, rowv[s]);
             Bits<NW> Fi[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
@@ -251,7 +260,7 @@ __device__ __forceinline__ void mbfs_pull(const GraphView& g, const MbfsArgs& a,
             }
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const uint32_t ms = hit[s] ? (m >> (8 * s)) & 0xFFu : 0u;
+                const uint32_t ms = hit[s] ? (m >> (8 * s)) & open : 0u;
 #pragma unroll
                 for (int b = 0; b < 8; ++b)
                     if ((ms >> b) & 1u)
@@ -275,11 +284,14 @@ __device__ __forceinline__ void mbfs_pull(const GraphView& g, const MbfsArgs& a,
         if (ok && sub < 8 && v < g.n && mine.any()) {
             const unsigned long long* rec = vis_at<NW>(a, v);
             unsigned long long* nr = next_at<NW>(a, v);
+            bool any = false;
 #pragma unroll
             for (int i = 0; i < NW; ++i) {
                 const unsigned long long nv = mine.w[i] & ~(rec[i] | rec[NW + i]);
                 if (nv) atomicOr(nr + i, nv);
+                any |= nv != 0;
             }
+            if (any) a.dirty[v >> 6] = 1;
         }
     }
 }
@@ -380,11 +392,14 @@ __device__ __forceinline__ void mbfs_td_mfma(const GraphView& g, const MbfsArgs&
 #pragma unroll
                 for (int i = 0; i < NW; ++i) seen[i] = rec[i] | rec[NW + i];
                 unsigned long long* nr = next_at<NW>(a, row);
+                bool any = false;
 #pragma unroll
                 for (int i = 0; i < NW; ++i) {
                     const unsigned long long nv = acc.w[i] & ~seen[i];
                     if (nv) atomicOr(nr + i, nv);
+                    any |= nv != 0;
                 }
+                if (any) a.dirty[row >> 6] = 1;
             }
         }
     }
@@ -490,11 +505,14 @@ mbfs_expand_kernel(GraphView g, MbfsArgs a, unsigned Lk) {
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         unsigned long long* nr = next_at<NW>(a, act[s] ? rowv[s] : 0u);
+                        bool any = false;
 #pragma unroll
                         for (int i = 0; i < NW; ++i) {
                             const unsigned long long nv = acc[s].w[i] & ~seen[s][i];
                             if (act[s] && nv) atomicOr(nr + i, nv);
+                            any |= act[s] && nv != 0;
                         }
+                        if (any) a.dirty[rowv[s] >> 6] = 1;  // a byte store: idempotent
                     }
                 }
             }
@@ -641,17 +659,24 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
     const Bits<NW> rootMask = root_mask<NW>(a.k);
     // groups of kGroup words: the group's records (visited and next, one line per vertex)
     // are all loaded before any is used, so a group costs one memory round trip
-    constexpr int kGroup = NW == 1 ? 4 : 2;  // (4 at 128 roots would cost the grid its residency)
+    constexpr int kGroup = NW == 1 ? 4 : 2;  // (4 at 128 roots spills past the 64 VGPRs of a resident grid)
+    // A word without a dirty mark holds no vertex whose `next` word was set this level:
+    // its records are not read, its frontier words are stored as zeros.
     for (uint32_t w0 = first; w0 < last; w0 += kGroup) {
         Bits<NW> nx[kGroup], vs[kGroup];
-        bool ok[kGroup];
+        bool ok[kGroup], dj[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; ++j) dj[j] = w0 + j < last && a.dirty[w0 + j] != 0;
 #pragma unroll
         for (int j = 0; j < kGroup; ++j) {
             const uint32_t v = (w0 + j) * 64 + lane_id();
             ok[j] = w0 + j < last && v < g.n;
-            const uint32_t vc = ok[j] ? v : 0u;
-            nx[j].loadp(next_at<NW>(a, vc));
-            vs[j].loadp(vis_at<NW>(a, vc));
+            nx[j].zero();
+            vs[j].zero();
+            if (ok[j] && dj[j]) {
+                nx[j].loadp(next_at<NW>(a, v));
+                vs[j].loadp(vis_at<NW>(a, v));
+            }
         }
         uint32_t deg[kGroup];
 #pragma unroll
@@ -691,11 +716,16 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
             if (lane_id() == 0 && w < last) {  // this wave owns word w here
                 if (fullBits) a.full[w] |= fullBits;
                 a.any[size_t(L & 1) * g.W + w] = anyBits;
+                if (dj[j]) a.dirty[w] = 0;
             }
         }
     }
-    __syncthreads();  // the frontier words above are read by other waves below
-    const unsigned running = segment_list<NW>(g, a.sv, fout, lout, l);
+    __syncthreads();  // the activity words above are read by other waves below
+    const unsigned long long* anyL = a.any + size_t(L & 1) * g.W;
+    const uint32_t ws2 = blockIdx.x * a.sv.wps, we2 = min(g.W, ws2 + a.sv.wps);
+    const unsigned running = block_write_vsets(
+        g.rsetUnit, ws2, we2, [](uint32_t) {}, [&](uint32_t w) { return anyL[w]; }, lout + a.sv.segBase[blockIdx.x],
+        l.ex, l.start, l.tmp);
     news = block_sum_u64(news, tmp64);
     edges = block_sum_u64(edges, tmp64);
     if (threadIdx.x == 0) {
@@ -734,13 +764,14 @@ __global__ void __launch_bounds__(kFinalBlock) mbfs_final_kernel(GraphView g, Mb
             const unsigned jn = min(64u, a.k - 64u * i);
             for (unsigned j = 0; j < jn; ++j) {
                 int32_t* out = a.levels + size_t(64 * i + j) * g.n + v;
+                // streaming stores: the k x n output is written once and not read here
                 if ((inHist >> j) & 1ull) {
                     int32_t l = 0;
 #pragma unroll
                     for (int b = 0; b < 5; ++b) l |= int32_t((plane[b] >> j) & 1ull) << b;
-                    *out = l;
+                    __builtin_nontemporal_store(l, out);
                 } else if (!((vis >> j) & 1ull)) {
-                    *out = -1;
+                    __builtin_nontemporal_store(int32_t(-1), out);
                 }
             }
         }
@@ -812,7 +843,8 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     DeviceGraph& g = *q.g;
     if (q.mbfsWords >= nw) return;
     for (void* p : {static_cast<void*>(q.mVis), static_cast<void*>(q.mFront0), static_cast<void*>(q.mFront1),
-                    static_cast<void*>(q.mHist), static_cast<void*>(q.mFull), static_cast<void*>(q.mAny)})
+                    static_cast<void*>(q.mHist), static_cast<void*>(q.mFull), static_cast<void*>(q.mAny),
+                    static_cast<void*>(q.mDirty)})
         q.release(p);
     // a new layout invalidates every captured multi-source level chunk
     for (int k : {kGraphMbfs1, kGraphMbfs2})
@@ -828,6 +860,7 @@ static void mbfs_ensure_state(Query& q, uint32_t nw) {
     q.mHist = q.alloc<unsigned long long>(words * (q.mHistLevels + 1));
     q.mFull = q.alloc<unsigned long long>(g.W);
     q.mAny = q.alloc<unsigned long long>(2 * size_t(g.W));
+    q.mDirty = q.alloc<uint8_t>(g.W);
     if (!q.srcs) q.srcs = q.alloc<uint32_t>(128);
     if (!q.rootStats) q.rootStats = q.alloc<dev::RootStats>(128);
     q.mbfsWords = nw;
@@ -884,6 +917,7 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     a.histLevels = q.mHistLevels;
     a.full = q.mFull;
     a.any = q.mAny;
+    a.dirty = q.mDirty;
     a.pull = g.symmetric ? 1 : 0;
     a.list0 = q.list0;
     a.list1 = q.list1;
