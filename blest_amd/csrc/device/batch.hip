@@ -272,6 +272,7 @@ blest_status blest_graph_batch_bfs(blest_graph_batch* h, const uint32_t* roots, 
         BLEST_HIP(hipSetDevice(gb.device));
         for (uint32_t i = 0; i < gb.count; ++i)
             if (roots[i] >= gb.sizes[i]) throw Error(BLEST_ERR_INVALID_ARGUMENT, "root out of range");
+        if (levels) check_result_range(levels, 4 * size_t(gb.totalN), gb.device, "levels");
         BLEST_HIP(hipMemcpyAsync(gb.roots, roots, 4 * size_t(gb.count), hipMemcpyHostToDevice, gb.stream));
         dev::BatchArgs a;
         a.graphs = gb.graphs;

@@ -636,6 +636,7 @@ blest_status blest_bfs(blest_query* qh, uint32_t src, int32_t* levels, blest_bfs
         Query& q = qh->q;
         std::lock_guard<std::mutex> lk(q.mu);
         BLEST_HIP(hipSetDevice(q.g->device));
+        if (levels) check_result_range(levels, sizeof(int32_t) * size_t(q.g->n), q.g->device, "levels");
         bfs_launch(q, map_source(*q.g, src));
         bfs_finish(q, levels, st);
     });
@@ -680,6 +681,7 @@ blest_status blest_bfs_trace(blest_query* qh, uint32_t src, int32_t* levels, ble
         Query& q = qh->q;
         std::lock_guard<std::mutex> lk(q.mu);
         BLEST_HIP(hipSetDevice(q.g->device));
+        if (levels) check_result_range(levels, sizeof(int32_t) * size_t(q.g->n), q.g->device, "levels");
         bfs_launch(q, map_source(*q.g, src), true);
         blest_bfs_stats s{};
         bfs_finish(q, levels, &s);

@@ -267,18 +267,10 @@ __global__ void __launch_bounds__(kProducerBlock) cc_lp_update_kernel(GraphView 
 }  // namespace dev
 
 // ------------------------------------------------------------------------ host
-static bool is_device_pointer(const void* p) {
-    hipPointerAttribute_t attr;
-    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-        (void)hipGetLastError();  // unregistered host memory
-        return false;
-    }
-    return attr.type == hipMemoryTypeDevice;
-}
-
 static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st) {
     DeviceGraph& g = *q.g;
     if (!g.symmetric) throw Error(BLEST_ERR_UNSUPPORTED, "connected components need a symmetric graph");
+    const bool labelsOnDevice = hostLabels && check_result_range(hostLabels, 4 * size_t(g.n), g.device, "labels");
     if (!q.labels) {
         q.labels = q.alloc<uint32_t>(g.n);
         q.labelsAux = q.alloc<uint32_t>(g.n);
@@ -338,7 +330,7 @@ static void cc_run(Query& q, int algo, uint32_t* hostLabels, blest_cc_stats* st)
     if (hostLabels) BLEST_HIP(hipMemcpyAsync(hostLabels, result, 4 * size_t(g.n), hipMemcpyDefault, q.stream));
     std::vector<uint32_t> tmp;
     const uint32_t* counted = hostLabels;
-    if (!hostLabels || is_device_pointer(hostLabels)) {
+    if (!hostLabels || labelsOnDevice) {
         tmp.resize(g.n);
         BLEST_HIP(hipMemcpyAsync(tmp.data(), result, 4 * size_t(g.n), hipMemcpyDeviceToHost, q.stream));
         counted = tmp.data();

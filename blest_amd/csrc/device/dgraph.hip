@@ -121,6 +121,30 @@ void run_levels(Query& q, const std::function<void(unsigned)>& level, const std:
     }
 }
 
+bool check_result_range(const void* dst, size_t bytes, int device, const char* what) {
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, dst) != hipSuccess) {
+        (void)hipGetLastError();  // unregistered (pageable) host memory
+        return false;
+    }
+    if (attr.type != hipMemoryTypeDevice) return false;
+    if (attr.device != device)
+        throw Error(BLEST_ERR_INVALID_ARGUMENT, std::string(what) + ": device memory of another device");
+    hipDeviceptr_t base = nullptr;
+    size_t extent = 0;
+    if (hipMemGetAddressRange(&base, &extent, const_cast<void*>(dst)) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(BLEST_ERR_INVALID_ARGUMENT, std::string(what) + ": not inside a device allocation");
+    }
+    const char* b = static_cast<const char*>(base);
+    const char* d = static_cast<const char*>(dst);
+    if (d < b || d + bytes > b + extent)
+        throw Error(BLEST_ERR_INVALID_ARGUMENT, std::string(what) + ": needs " + std::to_string(bytes) +
+                                                    " bytes, the device allocation holds " +
+                                                    std::to_string(size_t(b + extent - d)) + " from there");
+    return true;
+}
+
 uint32_t map_source(const DeviceGraph& g, uint32_t src) {
     if (src >= g.n) throw Error(BLEST_ERR_INVALID_ARGUMENT, "source vertex out of range");
     return g.hasPerm ? g.hostOldToNew[src] : src;

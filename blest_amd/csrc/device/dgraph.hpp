@@ -182,6 +182,13 @@ inline uint32_t mbfs_history_levels(uint32_t n, uint32_t nw) {
     const uint64_t fit = (uint64_t(32) << 30) / std::max<uint64_t>(perLevel, 1);
     return uint32_t(std::min<uint64_t>(16, std::max<uint64_t>(fit, 2) - 1));
 }
+// Result destinations.  A caller's array may be host memory or device memory of the
+// graph's device; when it is device memory, [dst, dst + bytes) must lie inside one
+// allocation (hipMemGetAddressRange), else BLEST_ERR_INVALID_ARGUMENT is thrown before
+// anything is copied: an undersized or wrong-graph buffer becomes an error, not an
+// out-of-bounds device write.  Host memory cannot be measured and is taken as given.
+// Returns whether dst is device memory.
+bool check_result_range(const void* dst, size_t bytes, int device, const char* what);
 // trace (nullable): record an event after the init kernel and after every level kernel.
 void bfs_launch(Query& q, uint32_t srcReordered, bool trace = false);
 void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st);

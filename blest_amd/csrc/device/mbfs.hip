@@ -904,6 +904,13 @@ static void mbfs_run(Query& q, const uint32_t* srcs, uint32_t k, int32_t* hostLe
     const int algo = o ? o->algo : 0;
     if (algo != 0 && algo != 1) throw Error(BLEST_ERR_INVALID_ARGUMENT, "algo must be 0 (bit-parallel) or 1 (i8 MFMA)");
     const uint32_t nw = (k > 64 || (o && o->batch > 64)) ? 2 : 1;
+    // result destinations are checked before anything runs
+    if (hostLevels) check_result_range(hostLevels, sizeof(int32_t) * size_t(g.n) * k, g.device, "levels");
+    if (rows)
+        for (uint32_t j = 0; j < k; ++j) {
+            if (!rows[j]) throw Error(BLEST_ERR_INVALID_ARGUMENT, "null result row");
+            check_result_range(rows[j], sizeof(int32_t) * size_t(g.n), g.device, "result row");
+        }
     mbfs_ensure_state(q, nw);
     std::vector<uint32_t> mapped(k);
     for (uint32_t j = 0; j < k; ++j) mapped[j] = map_source(g, srcs[j]);

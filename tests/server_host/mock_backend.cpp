@@ -12,6 +12,8 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <mutex>
 #include <random>
 #include <set>
@@ -73,6 +75,20 @@ void pause_us(uint32_t salt) {
 }
 
 uint64_t footprint(uint32_t cap) { return 1000 + 100ull * cap; }
+
+std::map<uintptr_t, size_t> gDevice;  // registered "device" buffers: start -> bytes (guarded by gMu)
+
+// A result copy of `bytes` to dst: when dst lies in a registered device buffer, the
+// whole copy must fit inside it.
+void check_dst(const void* dst, size_t bytes) {
+    std::lock_guard<std::mutex> lk(gMu);
+    const uintptr_t d = reinterpret_cast<uintptr_t>(dst);
+    auto it = gDevice.upper_bound(d);
+    if (it == gDevice.begin()) return;
+    --it;
+    if (d >= it->first + it->second) return;  // not inside a device buffer: host memory
+    if (d + bytes > it->first + it->second) violation("result copy overruns its device buffer");
+}
 }  // namespace
 
 // ------------------------------------------------------------------ driver hooks
@@ -84,6 +100,19 @@ blest_device_graph* mock_graph_create(uint32_t n, uint32_t symmetric, uint32_t t
     return g;
 }
 void mock_graph_free(blest_device_graph* g) { delete g; }
+void* mock_device_alloc(size_t bytes) {
+    void* p = std::malloc(bytes);
+    std::lock_guard<std::mutex> lk(gMu);
+    gDevice[reinterpret_cast<uintptr_t>(p)] = bytes;
+    return p;
+}
+void mock_device_free(void* p) {
+    {
+        std::lock_guard<std::mutex> lk(gMu);
+        if (!gDevice.erase(reinterpret_cast<uintptr_t>(p))) violation("unknown device buffer freed");
+    }
+    std::free(p);
+}
 uint64_t mock_footprint(uint32_t max_sources) { return footprint(max_sources); }
 void mock_set_budget(uint64_t bytes) {
     std::lock_guard<std::mutex> lk(gMu);
@@ -147,6 +176,7 @@ blest_status blest_bfs(blest_query* q, uint32_t src, int32_t* levels, blest_bfs_
     Use use(q);
     pause_us(src);
     if (src == kFailRoot) return blest::fail(BLEST_ERR_DEVICE, "planted failure");
+    check_dst(levels, sizeof(int32_t) * q->g->n);
     for (uint32_t i = 0; i < q->g->n; ++i) levels[i] = int32_t(src);
     if (st) {
         *st = blest_bfs_stats{};
@@ -159,6 +189,7 @@ blest_status blest_bfs(blest_query* q, uint32_t src, int32_t* levels, blest_bfs_
 blest_status blest_cc(blest_query* q, int, uint32_t* labels, blest_cc_stats* st) {
     Use use(q);
     pause_us(q->g->tag);
+    check_dst(labels, sizeof(uint32_t) * q->g->n);
     for (uint32_t i = 0; i < q->g->n; ++i) labels[i] = 0xC0000000u | q->g->tag;
     if (st) {
         *st = blest_cc_stats{};
@@ -177,6 +208,7 @@ blest_status mbfs_rows(blest_query* q, const uint32_t* srcs, uint32_t k, int32_t
     pause_us(k);
     for (uint32_t j = 0; j < k; ++j)
         if (srcs[j] == kFailRoot) return fail(BLEST_ERR_DEVICE, "planted failure");
+    for (uint32_t j = 0; j < k; ++j) check_dst(rows[j], sizeof(int32_t) * q->g->n);
     for (uint32_t j = 0; j < k; ++j)
         for (uint32_t i = 0; i < q->g->n; ++i) rows[j][i] = int32_t(srcs[j]);
     if (st) *st = blest_bfs_stats{};

@@ -1,6 +1,7 @@
 // Driver hooks of the host-only server backend (mock_backend.cpp).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include "blest.h"
@@ -20,3 +21,8 @@ void mock_graph_free(blest_device_graph* g);
 uint64_t mock_footprint(uint32_t max_sources);
 void mock_set_budget(uint64_t bytes);
 mock_report mock_report_get();
+// "Device" result buffers: host memory registered with its size, so the backend can
+// check that every result copy into one stays inside it (the library checks device
+// destinations with hipMemGetAddressRange the same way).
+void* mock_device_alloc(size_t bytes);
+void mock_device_free(void* p);

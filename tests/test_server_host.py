@@ -1,4 +1,5 @@
-"""The request server's host concurrency under ThreadSanitizer (CPU only).
+"""The request server's host concurrency under ThreadSanitizer, and its memory safety
+under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only).
 
 blest_amd/csrc/server.cpp is compiled unchanged with a host-only backend
 (tests/server_host/mock_backend.cpp) that stands in for the device queries and checks
@@ -6,7 +7,11 @@ the server's contract with them: one thread per query at a time, no use after fr
 live query bytes within the HBM budget.  tests/server_host/stress.cpp drives it from
 four client threads (BFS and CC on three graphs, planted traversal failures, a full
 request queue, polling waits, a small budget that forces pool eviction and waits, and a
-free with requests still queued) and checks every result.  ROCm's clang builds it: the
+free with requests still queued) and checks every result.  A second phase has the serving benchmark's shape (4 workers,
+64-root batches, CC requests, results written into registered "device" buffers sized
+for each graph's n, eviction under a small budget); the backend checks that every result
+copy stays inside its buffer, as the library checks device destinations.  ROCm's clang
+builds the ThreadSanitizer binary: the
 ThreadSanitizer runtime of GCC 11 reports a false "double lock" inside
 condition-variable waits on this system."""
 import os
@@ -43,3 +48,29 @@ def test_server_concurrency_under_thread_sanitizer(tmp_path):
     assert r.returncode == 0, out[-6000:]
     assert "server stress ok" in r.stdout
     assert "ThreadSanitizer" not in out
+
+
+def _asan_build(tmp_path: Path) -> Path:
+    exe = tmp_path / "server_stress_asan"
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-pthread",
+           f"-I{ROOT / 'include'}", f"-I{ROOT / 'blest_amd' / 'csrc'}", f"-I{SRC}",
+           str(ROOT / "blest_amd" / "csrc" / "server.cpp"), str(SRC / "mock_backend.cpp"), str(SRC / "stress.cpp"),
+           "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    return exe
+
+
+def test_server_memory_safety_under_address_sanitizer(tmp_path):
+    exe = _asan_build(tmp_path)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([str(exe), "1500"], capture_output=True, text=True, timeout=240, env=env)
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out[-6000:]
+    assert "server stress ok" in r.stdout
+    assert "runtime error" not in out and "AddressSanitizer" not in out
+    # a planted mistake (device buffers one entry short) is caught
+    bad = subprocess.run([str(exe), "200", "short"], capture_output=True, text=True, timeout=240, env=env)
+    bout = bad.stdout + bad.stderr
+    assert bad.returncode != 0
+    assert "overruns its device buffer" in bout or "heap-buffer-overflow" in bout
