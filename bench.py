@@ -467,7 +467,7 @@ def main():
     import blest_amd as B
     from blest_amd import workloads as W
 
-    prep = W.prepare(a.workload, log=log)
+    prep = W.prepare(a.workload, log=log, device_bvss=True)
     w = prep.workload
     opts = {}
     if a.block_size:

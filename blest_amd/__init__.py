@@ -202,6 +202,16 @@ class BVSS(BitMatrix):
         self._h = h
         return self
 
+    def construct_on_device(self, csc: CSC, full_padding: bool | None = None, device: int = 0) -> "BVSS":
+        """The same arrays built on the GPU (blest_bvss_build_device)."""
+        h = C.c_void_p()
+        fp = -1 if full_padding is None else int(full_padding)
+        check(self._lib.blest_bvss_build_device(csc._h, self.slice_size, self.no_masks, fp, device, C.byref(h)))
+        if self._h.value:
+            self._lib.blest_bvss_free(self._h)
+        self._h = h
+        return self
+
     @classmethod
     def load_binary(cls, path: str) -> "BVSS":
         b = cls()

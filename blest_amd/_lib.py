@@ -116,6 +116,7 @@ PROTOTYPES = {
     "blest_ordering_save": (C.c_int, [C.c_char_p, U32P, C.c_uint32]),
     "blest_ordering_load": (C.c_int, [C.c_char_p, U32P, C.c_uint32]),
     "blest_bvss_build": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int, PP]),
+    "blest_bvss_build_device": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_int, C.c_int, PP]),
     "blest_bvss_free": (None, [P]),
     "blest_bvss_info_get": (C.c_int, [P, C.POINTER(BvssInfo)]),
     "blest_bvss_export": (C.c_int, [P, U64P, U32P, U32P, U32P, U32P]),

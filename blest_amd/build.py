@@ -23,7 +23,7 @@ ARCH = os.environ.get("BLEST_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["csc.cpp", "reorder.cpp", "bvss.cpp", "capi_host.cpp", "server.cpp", "autotune.cpp"]
 DEVICE_SOURCES = ["device/dgraph.hip", "device/bfs.hip", "device/cc.hip", "device/mbfs.hip", "device/batch.hip",
-                  "device/bfs_noalloc.hip"]
+                  "device/bfs_noalloc.hip", "device/bvss_build.hip"]
 CLI_SOURCE = "cli/blest_main.cpp"
 
 COMMON = ["-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROOT / 'include'}", f"-I{CSRC}"]

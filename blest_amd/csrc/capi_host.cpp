@@ -25,6 +25,7 @@ blest_status fail(blest_status s, const std::string& msg) {
 
 const CSC& csc_of(const blest_csc* h) { return h->g; }
 const BVSS& bvss_of(const blest_bvss* h) { return h->b; }
+blest_bvss* bvss_wrap(BVSS&& b) { return new blest_bvss{std::move(b)}; }
 
 }  // namespace blest
 

@@ -16,6 +16,8 @@ extern thread_local std::string tl_last_error;
 blest_status fail(blest_status s, const std::string& msg);
 const CSC& csc_of(const blest_csc* h);
 const BVSS& bvss_of(const blest_bvss* h);
+// A new handle owning b.
+blest_bvss* bvss_wrap(BVSS&& b);
 
 // Multi-source BFS writing the levels of root j into rows[j] (n entries each); used by
 // the request server to hand each request its own row without a staging copy.

@@ -137,7 +137,9 @@ class Prepared:
     seconds: dict
 
 
-def prepare(name: str, keep_original: bool = False, log=print) -> Prepared:
+def prepare(name: str, keep_original: bool = False, log=print, device_bvss: bool = False) -> Prepared:
+    """device_bvss: build the BVSS on the GPU (blest_bvss_build_device, the same arrays
+    as the host construction) instead of on the host."""
     w = WORKLOADS[name]
     secs = {}
     t = time.time()
@@ -153,8 +155,11 @@ def prepare(name: str, keep_original: bool = False, log=print) -> Prepared:
     inv = reorder_cached(g, w, log)
     secs["reorder"] = time.time() - t
     t = time.time()
-    b = BVSS(8, 4).construct_from_csc(g, full_padding=not social)  # Benchmark.cuh:187-194
+    if device_bvss:
+        b = BVSS(8, 4).construct_on_device(g, full_padding=not social)
+    else:
+        b = BVSS(8, 4).construct_from_csc(g, full_padding=not social)  # Benchmark.cuh:187-194
     secs["bvss"] = time.time() - t
-    log(f"[prep] bvss: {b.info()['no_slices']} slices, {b.info()['no_virtual_slice_sets']} vsets "
-        f"({secs['bvss']:.1f}s)")
+    log(f"[prep] bvss ({'device' if device_bvss else 'host'}): {b.info()['no_slices']} slices, "
+        f"{b.info()['no_virtual_slice_sets']} vsets ({secs['bvss']:.1f}s)")
     return Prepared(w, g, inv, b, cp0, rows0, roots, social, secs)

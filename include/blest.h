@@ -142,6 +142,14 @@ typedef struct blest_bvss blest_bvss;
  * pass -1 to derive it from the CSC's social classification. */
 blest_status blest_bvss_build(const blest_csc* g, uint32_t slice_size, uint32_t no_masks,
                               int full_padding, blest_bvss** out);
+/* The same construction on the GPU (slice size 8 with 4 masks only, else
+ * BLEST_ERR_UNSUPPORTED): a segmented radix sort of the (row, column bit) keys of each
+ * rset, slices numbered by a scan, vsets laid out by prefix sums and a scatter; the
+ * arrays (and the update divergence) are bit-identical to blest_bvss_build's and are
+ * returned in host memory.  device: HIP device ordinal.  Reference:
+ * BVSS::constructFromCSCMatrix (BVSS.cuh:185-467), which the reference runs on the host. */
+blest_status blest_bvss_build_device(const blest_csc* g, uint32_t slice_size, uint32_t no_masks,
+                                     int full_padding, int device, blest_bvss** out);
 void blest_bvss_free(blest_bvss* b);
 
 typedef struct blest_bvss_info {
