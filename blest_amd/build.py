@@ -73,6 +73,10 @@ def build(jobs: int = 0, debug: bool = False, force: bool = True) -> Path:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(tmp), "-pthread",
           "-Wl,-soname,libblest.so"])
     os.replace(tmp, lib)
+    # measurement probe of level-synchronisation costs (no library code)
+    probe = CSRC / "probes" / "level_sync_probe.hip"
+    if probe.exists():
+        _run([HIPCC, "-std=c++20", *opt, *HIP_FLAGS, str(probe), "-o", str(PKG / "level_sync_probe")])
     cli_src = CSRC / CLI_SOURCE
     if cli_src.exists():
         exe = PKG / "blest"
