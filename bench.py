@@ -420,6 +420,8 @@ def main():
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
     ap.add_argument("--td-quads", type=int, default=0, help="top-down slice tile: quads per lane per step (1, 2, 4)")
     ap.add_argument("--bu-group", type=int, default=0, help="bottom-up staging depth: words probed together (4, 8)")
+    ap.add_argument("--fuse-quads", type=int, default=0,
+                    help="quads a small level may stream fused in one workgroup (0 default, < 0 never)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick blocks_per_cu / alpha / beta with the native autotuner on --autotune-roots "
                          "of the workload's roots before timing (default on; --no-autotune keeps the "
@@ -482,6 +484,8 @@ def main():
         opts["direction"] = a.direction
     if a.use_graphs >= 0:
         opts["use_graphs"] = a.use_graphs
+    if a.fuse_quads:
+        opts["fuse_quads"] = a.fuse_quads
     if a.mbfs:
         print(json.dumps(mbfs_bench(a, prep, opts, algos=tuple(a.mbfs_algo))), flush=True)
         return

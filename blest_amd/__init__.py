@@ -315,6 +315,11 @@ class DeviceGraph:
     def set_thresholds(self, alpha: float, beta: float, direction: int = 0) -> None:
         check(self._lib.blest_device_graph_set_thresholds(self._h, alpha, beta, direction))
 
+    def set_fuse(self, fuse_quads: int) -> None:
+        """Quads a small top-down level may stream inside one workgroup (levels fused with no
+        kernel boundary between them); 0 = default (2048), < 0 = never."""
+        check(self._lib.blest_device_graph_set_fuse(self._h, fuse_quads))
+
 
 def autotune_bfs(bvss: BVSS, csc: CSC | None, inv_perm, roots, **base) -> tuple[dict, float, int]:
     """Native autotuner: returns (best device opts, their mean BFS ms, candidates timed)."""

@@ -42,7 +42,7 @@ class DeviceOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("alpha", C.c_double), ("beta", C.c_double), ("direction", C.c_int), ("keep_csr", C.c_int),
                 ("timeout_ms", C.c_uint32), ("use_graphs", C.c_int), ("td_quads", C.c_uint32),
-                ("bu_group", C.c_uint32)]
+                ("bu_group", C.c_uint32), ("fuse_quads", C.c_int32)]
 
 
 class DeviceGraphInfo(C.Structure):
@@ -144,6 +144,7 @@ PROTOTYPES = {
     "blest_cc": (C.c_int, [P, C.c_int, U32P, C.POINTER(CcStats)]),
     "blest_device_graph_set_thresholds": (C.c_int, [P, C.c_double, C.c_double, C.c_int]),
     "blest_device_graph_set_tiles": (C.c_int, [P, C.c_uint32, C.c_uint32]),
+    "blest_device_graph_set_fuse": (C.c_int, [P, C.c_int32]),
     "blest_autotune_bfs": (C.c_int, [P, P, U32P, U32P, C.c_uint32, C.POINTER(DeviceOpts), C.POINTER(DeviceOpts),
                                      DP, U32P]),
     "blest_host_alloc": (C.c_int, [C.c_size_t, PP]),

@@ -53,6 +53,7 @@ struct BfsArgs {
     int direction;
     double alpha, beta;
     int levelFromCtrl;  // 1: the kernel's L argument is relative to ctrl->levelBase
+    unsigned fuseQuads;  // quads a fused small level may stream (0: levels never fused)
 };
 
 // a[k] for a per-lane k (a select chain: the array stays in registers); 0 for k >= N.
@@ -88,6 +89,7 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
         const unsigned long long x = (w == sw) ? sb : 0ull;
         a.visited[w] = x;
         a.front1[w] = x;
+        a.front0[w] = 0ull;  // a level's output bitmap starts empty
     }
     unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
     for (size_t i = tid; i < size_t(g.W) * 8; i += nthreads) nb64[i] = 0ull;
@@ -121,6 +123,328 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
     }
 }
 
+// ------------------------------------------------------------ fused small levels
+// A top-down level whose lists hold a few vsets (a root's first levels, the last levels of
+// a traversal, the first levels of a deep one) costs 10-25 us in the two-kernel form: two
+// kernel boundaries, a pass over every bitmap word and a few dependent memory round trips
+// in each kernel, for a few hundred slices.  bfs_fused_kernel, one workgroup, runs such
+// levels back to back with no kernel boundary between them.  It is enqueued after the init
+// kernel and after each chunk of level kernels; kernel argument Lk is the enqueued level
+// whose inputs it reads (lists, counters, input bitmap, state slot), and when it stops it
+// leaves the inputs of that same enqueued level, Lk's state carrying the levels it ran as
+// its shift, so the expand kernel enqueued as Lk goes on from there.  Per level:
+//   queue   the frontier's rsets, in the top-down discovery bytes `nextB` (all zero
+//           between levels; two queues of up to R entries, zeroed again after use): the
+//           first level's from its lists (the first vset of each listed rset), later
+//           ones from the discoveries.  A level runs here while its queue holds at most
+//           kFuseTile rsets and fuseQuads quads (an rset's quads are contiguous);
+//   stream  the queue's quads, kFuseQpl per thread per pass, the owner entry found by
+//           binary search in an LDS prefix of the entries' quad counts; a slice hit on a
+//           row not yet visited claims the row with a 32-bit atomicOr on `visited`; the
+//           winner writes the level, sets the row's bit in the output bitmap, and when
+//           that atomic's old value shows no bit of the row's rset yet, appends the rset;
+//   clear   the input bitmap's words of the queue's rsets, so the buffer is empty again
+//           when it is the output bitmap two levels on.
+// A level's output bitmap starts empty: the init kernel clears it for level 1 and every
+// update kernel clears the input bitmap it consumed.  A traversal that ends here raises
+// the done flag as level_begin does.  Otherwise, when the next level is too big, bottom-up
+// by the direction rule, or the schedule is bottom-up after level 1, the workgroup writes
+// the next level's lists (in the update kernel's form, from the start of the list arrays:
+// one flat range that segment 0's counters describe) and moves its frontier bits into the
+// input bitmap.  Bits of rsets without vsets stay in the first input bitmap (no list names
+// them): their vertices have no out-edges, so no direction ever tests them.
+constexpr int kFusedBlock = 1024;
+constexpr unsigned kFuseTile = 2 * kFusedBlock;  // rsets a fused level's queue may hold
+constexpr int kFuseQpl = 4;                      // quads per thread per pass
+// While the frontier shrinks (the last levels of a traversal) a level discovers few rows,
+// so levels of up to kFuseShrinkScale x fuseQuads quads still run here.
+constexpr unsigned kFuseShrinkScale = 4;
+constexpr uint32_t kNoRset = 0xFFFFFFFFu;
+typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+// A queue of rsets: entries below kFuseTile in LDS (the level prefix arrays, free once the
+// first level's lists are read), later ones in global memory at the same index (the
+// top-down discovery bytes `nextB`, all zero between levels: two queues of up to R
+// entries fit in its 64 W bytes; an entry used there is zeroed again after use).
+struct FusedQueue {
+    uint32_t* lds;
+    uint32_t* glb;
+    __device__ uint32_t get(unsigned i) const { return i < kFuseTile ? lds[i] : glb[i]; }
+    __device__ void put(unsigned i, uint32_t r) const {
+        if (i < kFuseTile)
+            lds[i] = r;
+        else
+            glb[i] = r;
+    }
+    __device__ void release(unsigned i) const {
+        if (i >= kFuseTile) glb[i] = 0u;
+    }
+};
+static_assert(kFuseTile <= unsigned(kMaxSegs), "an LDS queue lives in a level prefix array");
+
+// Listed vsets (<= 32 quads each) with which a level enters the fused kernel: the entering
+// level never exceeds fuseQuads, and holds at most one list entry per thread.
+__host__ __device__ constexpr unsigned fuse_vsets(unsigned fuseQuads) {
+    return fuseQuads / 32 < unsigned(kFusedBlock) ? fuseQuads / 32 : unsigned(kFusedBlock);
+}
+
+// Entries 2t, 2t + 1 of queue q (nq <= kFuseTile entries): exclusive quad prefix into pf,
+// first quad into q0, frontier byte (replicated to the 4 slots of a quad) into fbq;
+// returns the queue's quad total.  Every thread must call it.
+__device__ __forceinline__ unsigned fused_queue_info(const GraphView& g, const FusedQueue& q, unsigned nq,
+                                                     const unsigned long long* front, uint32_t* pf, uint32_t* q0,
+                                                     uint32_t* fbq, unsigned* tmp32) {
+    const unsigned t = threadIdx.x;
+    uint32_t u0[2], u1[2], fb[2], qs[2], cnt[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const unsigned i = 2 * t + k;
+        const bool ok = i < nq;
+        const uint32_t r = ok ? q.lds[i] : 0u;
+        u0[k] = ok ? g.rsetUnit[r] : 0u;
+        u1[k] = ok ? g.rsetUnit[r + 1] : 0u;
+        fb[k] = ok ? front_byte(front, r) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool live = fb[k] && u1[k] > u0[k];
+        qs[k] = live ? g.unitQuad[u0[k]] : 0u;
+        cnt[k] = live ? g.unitQuad[u1[k]] - qs[k] : 0u;
+    }
+    unsigned total;
+    const unsigned ex = block_exclusive_scan(cnt[0] + cnt[1], tmp32, total);
+    pf[2 * t] = ex;
+    pf[2 * t + 1] = ex + cnt[0];
+    q0[2 * t] = qs[0];
+    q0[2 * t + 1] = qs[1];
+    fbq[2 * t] = fb[0] * 0x01010101u;
+    fbq[2 * t + 1] = fb[1] * 0x01010101u;
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(total);
+}
+
+// Lists of queue entries [0, nq) (2 per thread per step), in the order and form the
+// update kernel gives them (an rset's vsets, or its chunks beyond kChunkVsets vsets),
+// appended at out + ltot / cout + ctot; ltot / ctot advance.  Every thread must call it.
+__device__ __forceinline__ void fused_write_lists(const GraphView& g, const FusedQueue& q, unsigned nq, uint32_t* out,
+                                                  uint2* cout, unsigned* tmp32, unsigned& ltot, unsigned& ctot) {
+    const unsigned t = threadIdx.x, nb = blockDim.x;
+    for (unsigned base = 0; base < nq; base += 2 * nb) {
+        uint32_t u0[2], cnt[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const unsigned i = base + 2 * t + k;
+            const uint32_t r = i < nq ? q.get(i) : 0u;
+            u0[k] = i < nq ? g.rsetUnit[r] : 0u;
+            cnt[k] = i < nq ? g.rsetUnit[r + 1] - u0[k] : 0u;
+        }
+        const uint32_t ch0 = rset_chunks(cnt[0]), ch1 = rset_chunks(cnt[1]);
+        const uint32_t l0 = ch0 ? 0u : cnt[0], l1 = ch1 ? 0u : cnt[1];
+        unsigned lsum, csum;
+        const unsigned lex = ltot + block_exclusive_scan(l0 + l1, tmp32, lsum);
+        const unsigned cex = ctot + block_exclusive_scan(ch0 + ch1, tmp32, csum);
+        for (uint32_t j = 0; j < l0; ++j) out[lex + j] = u0[0] + j;
+        for (uint32_t j = 0; j < l1; ++j) out[lex + l0 + j] = u0[1] + j;
+        for (uint32_t j = 0; j < ch0; ++j)
+            cout[cex + j] = make_uint2(u0[0] + j * kChunkVsets, min(kChunkVsets, cnt[0] - j * kChunkVsets));
+        for (uint32_t j = 0; j < ch1; ++j)
+            cout[cex + ch0 + j] = make_uint2(u0[1] + j * kChunkVsets, min(kChunkVsets, cnt[1] - j * kChunkVsets));
+        ltot += __builtin_amdgcn_readfirstlane(lsum);
+        ctot += __builtin_amdgcn_readfirstlane(csum);
+    }
+}
+
+__global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+    const unsigned Lin = level_of(a.ctrl, a.levelFromCtrl, Lk);
+    __shared__ unsigned pre[kMaxSegs], cpre[kMaxSegs];
+    __shared__ uint32_t pf[kFuseTile], q0[kFuseTile], fbq[kFuseTile];
+    __shared__ unsigned cntr, tmp32[16];
+    __shared__ unsigned long long tmp64[16];
+    Ctrl* c = a.ctrl;
+    const uint32_t segs = a.sv.segs;
+    const unsigned t = threadIdx.x, nb = blockDim.x;
+    DirectionParams dp;
+    dp.direction = a.direction;
+    dp.canBottomUp = g.csrPtr != nullptr;
+    dp.alpha = a.alpha;
+    dp.beta = a.beta;
+    dp.totalEdges = g.nnz;
+    dp.totalItems = double(g.n);
+    bool bottomUp = false;
+    unsigned total = 0, ctotal = 0, Le = 0;
+    LevelState cur, in;  // state of the level about to be expanded, and of the one before
+    if (!level_begin(c, a.sc, segs, Lin, dp, bottomUp, pre, &total, cpre, &ctotal, &Le, &cur, &in)) return;
+    // too big or bottom-up: the expand kernel enqueued as Lin takes it (its state slot
+    // written above is rewritten there from the same inputs)
+    const unsigned scale = cur.prevNf < in.prevNf ? kFuseShrinkScale : 1u;
+    if (bottomUp || total + ctotal * kChunkVsets > fuse_vsets(scale * a.fuseQuads)) return;
+    if (t == 0 && !c->tailFrom && Lin > 1) c->tailFrom = Lin;
+    uint32_t* lists = (Lin & 1) ? a.list1 : a.list0;  // read first, rewritten when handing back
+    uint2* chunks = (Lin & 1) ? a.chunk1 : a.chunk0;
+    unsigned long long* fin = (Lin & 1) ? a.front1 : a.front0;  // the frontier ends here
+    // the first level's rsets: list entry t and chunk entry t (at most one each)
+    uint32_t cand0 = kNoRset, cand1 = kNoRset;
+    if (t < total) {
+        const uint32_t v = list_entry(lists, a.sv, pre, t);
+        const uint32_t r = g.unitRset[v], rp = v ? g.unitRset[v - 1] : kNoRset;
+        if (rp != r) cand0 = r;
+    }
+    if (t < ctotal) {
+        const uint2 ch = chunk_entry(chunks, a.sv, cpre, t);
+        const uint32_t r = g.unitRset[ch.x], rp = ch.x ? g.unitRset[ch.x - 1] : kNoRset;
+        if (rp != r) cand1 = r;
+    }
+    FusedQueue qc{pre, reinterpret_cast<uint32_t*>(a.nextB)};
+    FusedQueue qn{cpre, reinterpret_cast<uint32_t*>(a.nextB) + g.R};
+    if (t == 0) cntr = 0;
+    __syncthreads();  // the list prefixes are read: they become the queues
+    if (cand0 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand0;  // <= 1024 + 64 entries
+    if (cand1 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand1;
+    __syncthreads();
+    unsigned nq = __builtin_amdgcn_readfirstlane(cntr);
+    unsigned long long* X = fin;                                  // input bitmap
+    unsigned long long* Y = (Lin & 1) ? a.front0 : a.front1;      // output bitmap (empty)
+    uint32_t* vis32 = reinterpret_cast<uint32_t*>(a.visited);
+    unsigned Lcur = Le;
+    LevelState prev = cur;  // state of level Lcur - 1 (from the second level on)
+    unsigned long long prevEdges = 0;  // degree sum of level Lcur's frontier
+    for (unsigned it = 0;; ++it) {
+        const unsigned T = nq <= kFuseTile ? fused_queue_info(g, qc, nq, X, pf, q0, fbq, tmp32) : 0xFFFFFFFFu;
+        if (it > 0) {
+            // the entering level always runs here; a later one only while small and top-down
+            const unsigned cap = cur.prevNf < prev.prevNf ? kFuseShrinkScale * a.fuseQuads : a.fuseQuads;
+            bool stop = nq > kFuseTile || T > cap || a.direction == 2;
+            if (!stop && a.direction == 0 && g.csrPtr) {
+                const double mu = double(g.nnz - min(prev.cumEdges + prevEdges, g.nnz));
+                stop = double(prevEdges) * a.alpha > mu && cur.prevNf > prev.prevNf;
+            }
+            if (stop) {
+                // level Lcur for the expand kernel enqueued as Lin: its lists, counters,
+                // frontier bitmap and the state of level Lcur - 1 with the shift
+                unsigned ltot = 0, ctot = 0;
+                fused_write_lists(g, qc, nq, lists, chunks, tmp32, ltot, ctot);
+                if (X != fin) {
+                    for (unsigned i = t; i < nq; i += nb) fin[qc.get(i) >> 3] = X[qc.get(i) >> 3];
+                    __syncthreads();  // a word shared by two entries is copied before either clears it
+                    for (unsigned i = t; i < nq; i += nb) X[qc.get(i) >> 3] = 0ull;
+                }
+                for (unsigned i = t; i < nq; i += nb) qc.release(i);
+                if (t == 0) {
+                    LevelState o = prev;
+                    o.shift = Lcur - Lin;
+                    c->state[(Lin - 1) & 1] = o;
+                }
+                const size_t base = size_t(Lin % 3) * segs;
+                for (uint32_t s = t; s < segs; s += nb) {
+                    a.sc.count[base + s] = s ? 0u : ltot;
+                    a.sc.chunks[base + s] = s ? 0u : ctot;
+                    a.sc.news[base + s] = s ? 0u : cur.prevNf;
+                    a.sc.edges[base + s] = s ? 0ull : prevEdges;
+                }
+                return;
+            }
+        }
+        if (t == 0) cntr = 0;  // every read of it came before the info's barriers
+        __syncthreads();
+        unsigned news = 0;
+        unsigned long long edges = 0;
+        uint32_t* y32 = reinterpret_cast<uint32_t*>(Y);
+        for (unsigned t0 = 0; t0 < T; t0 += nb * kFuseQpl) {
+            uint32_t row[4 * kFuseQpl], w[4 * kFuseQpl];
+            unsigned hit = 0;  // bit s: slice s hits the frontier
+#pragma unroll
+            for (int h = 0; h < kFuseQpl; ++h) {
+                const unsigned tt = t0 + h * nb + t;
+                uint4 r4 = make_uint4(0, 0, 0, 0);
+                uint32_t m = 0;
+                if (tt < T) {
+                    unsigned o = 0;  // the last entry whose prefix is <= tt
+#pragma unroll
+                    for (unsigned step = kFuseTile / 2; step > 0; step >>= 1)
+                        if (o + step < nq && pf[o + step] <= tt) o += step;
+                    const uint32_t q = q0[o] + (tt - pf[o]);
+                    r4 = stream_load_quad(g.rows, q);
+                    m = stream_load_u32(g.masks, q) & fbq[o];
+                }
+                row[4 * h + 0] = r4.x;
+                row[4 * h + 1] = r4.y;
+                row[4 * h + 2] = r4.z;
+                row[4 * h + 3] = r4.w;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((m >> (8 * j)) & 0xFFu) hit |= 1u << (4 * h + j);
+            }
+            // not yet visited (a stale word only shows fewer bits: the atomic decides)
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s) w[s] = ((hit >> s) & 1u) ? vis32[row[s] >> 5] : 0u;
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s)
+                if ((w[s] >> (row[s] & 31)) & 1u) hit &= ~(1u << s);
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s)
+                w[s] = ((hit >> s) & 1u) ? atomicOr(&vis32[row[s] >> 5], 1u << (row[s] & 31)) : 0u;
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s)
+                if ((w[s] >> (row[s] & 31)) & 1u) hit &= ~(1u << s);
+            // the rows claimed here
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s) {
+                if ((hit >> s) & 1u) {
+                    a.levels[row[s]] = int32_t(Lcur);
+                    w[s] = atomicOr(&y32[row[s] >> 5], 1u << (row[s] & 31));
+                    if (g.csrPtr) {  // both list bounds in one 8-byte load
+                        const u32x2_a4 d = *reinterpret_cast<const u32x2_a4*>(g.csrPtr + row[s]);
+                        edges += d.y - d.x;
+                    }
+                    news += 1;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 4 * kFuseQpl; ++s)
+                if (((hit >> s) & 1u) && ((w[s] >> (row[s] & 24)) & 0xFFu) == 0u)
+                    qn.put(atomicAdd(&cntr, 1u), row[s] >> 3);  // first vertex of its rset
+        }
+        const unsigned nf = __builtin_amdgcn_readfirstlane(unsigned(block_sum_u64(news, tmp64)));
+        const unsigned long long eSum = block_sum_u64(edges, tmp64);  // the sums' barriers follow every append
+        const unsigned long long mf = (unsigned long long)__builtin_amdgcn_readfirstlane(unsigned(eSum >> 32)) << 32 |
+                                      __builtin_amdgcn_readfirstlane(unsigned(eSum));
+        const unsigned nNew = __builtin_amdgcn_readfirstlane(cntr);
+        // this level's frontier words and queue entries are no longer read
+        for (unsigned i = t; i < nq; i += nb) {  // nq <= kFuseTile: all in LDS
+            X[qc.lds[i] >> 3] = 0ull;
+        }
+        if (nf == 0) {
+            if (t == 0) {
+                c->levels = Lcur;
+                c->fusedLevels = Lcur - (Lin - 1);
+                c->tdLevels = cur.td;
+                c->buLevels = cur.bu;
+                c->visited = cur.visited;
+                c->visitedEdges = cur.cumEdges;
+                c->buMask = cur.buMask;
+                __hip_atomic_store(&c->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+        prev = cur;
+        cur.dir = 0;
+        cur.prevNf = nf;
+        cur.visited = prev.visited + nf;
+        cur.td = prev.td + 1;
+        cur.cumEdges = prev.cumEdges + mf;
+        prevEdges = mf;
+        ++Lcur;
+        const FusedQueue tq = qc;
+        qc = qn;
+        qn = tq;
+        nq = nNew;
+        unsigned long long* tb = X;
+        X = Y;
+        Y = tb;
+        __syncthreads();  // the cleared words and queue entries precede the next level's use
+    }
+}
+
 // Two launch-time knobs, both chosen by the autotuner: QPL = quads each lane streams per
 // top-down step (the slice tile a wave holds in flight: 64 x QPL quads = 256 x QPL
 // slices), BUG = bitmap words a bottom-up wave stages per group (vertices whose first
@@ -142,7 +466,15 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     bool bottomUp = false;
     __shared__ unsigned cpre[kMaxSegs];
     unsigned total = 0, ctotal = 0;  // list entries / chunks of the level (top-down)
-    if (!level_begin(c, a.sc, segs, L, dp, bottomUp, pre, &total, cpre, &ctotal)) return;
+    unsigned Le;                     // the level's distance
+    LevelState st, in;
+    if (!level_begin(c, a.sc, segs, L, dp, bottomUp, pre, &total, cpre, &ctotal, &Le, &st, &in)) return;
+    if (a.fuseQuads && blockIdx.x == 0 && threadIdx.x == 0) {
+        // where the next traversal's fused kernel should take over (the level driver's hint)
+        const bool small = !bottomUp && st.prevNf < in.prevNf &&
+                           total + ctotal * kChunkVsets <= fuse_vsets(kFuseShrinkScale * a.fuseQuads);
+        c->tailFrom = small ? (c->tailFrom ? c->tailFrom : L) : 0u;
+    }
     const unsigned lane = lane_id();
     const unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
@@ -259,7 +591,8 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
 #pragma unroll
         for (int k = 0; k < kBuGroup; ++k) {
             const uint32_t w = w0 + k * wpb;
-            const unsigned long long vis = w < we ? a.visited[w] : ~0ull;
+            // a vertex without in-neighbours is never found bottom-up: closed like a visited one
+            const unsigned long long vis = w < we ? (a.visited[w] | (g.noIn ? g.noIn[w] : 0ull)) : ~0ull;
             const uint32_t u = w * 64 + lane;
             const bool open = w < we && u < g.n && !((vis >> lane) & 1ull);
             pb[k] = open ? g.csrPtr[u] : 0u;
@@ -335,7 +668,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                     fout[w] = nw;
                 }
                 if (f) {
-                    a.levels[w * 64 + lane] = int32_t(L);
+                    a.levels[w * 64 + lane] = int32_t(Le);
                     news += 1;
                     edges += pe[k] - pb[k];
                 }
@@ -356,7 +689,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             const unsigned long long bit = 1ull << (u & 63);
             atomicOr(&a.visited[u >> 6], bit);
             atomicOr(&fout[u >> 6], bit);
-            a.levels[u] = int32_t(L);
+            a.levels[u] = int32_t(Le);
             news += 1;
             edges += p1 - p0;
         }
@@ -384,6 +717,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     __shared__ uint32_t sStart[kProducerBlock];
     __shared__ unsigned long long sWord[kProducerBlock / 8];
     Ctrl* c = a.ctrl;
+    unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
@@ -392,13 +726,14 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     // one batch of independent loads: the flags, the segment's list bases and both
     // candidate words of the first chunk (next for top-down, fout for bottom-up)
     const unsigned done = ld_relaxed(&c->done);
-    const unsigned dir = c->state[L & 1].dir;
+    const unsigned dir = c->state[L & 1].dir, shift = c->state[L & 1].shift;
     const uint32_t listBase = a.sv.segBase[blockIdx.x], chunkBase = a.sv.chunkBase[blockIdx.x];
     const uint32_t w1 = min(ws + t / 8, g.W - 1);
     unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
     const unsigned long long firstNB = nb64[size_t(w1) * 8 + b], firstFout = fout[w1];
     if (done) return;
     const bool bottomUp = dir != 0;
+    const unsigned Le = L + shift;  // the level's distance
     uint32_t* out = ((L & 1) ? a.list0 : a.list1) + listBase;
     uint2* cout = ((L & 1) ? a.chunk0 : a.chunk1) + chunkBase;
     const bool canBU = g.csrPtr != nullptr;
@@ -424,11 +759,15 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             if (raw) nb64[size_t(w) * 8 + b] = 0ull;
         }
         if (w < we) {
+            // the input bitmap is consumed: emptied for the level after next, whose output
+            // it is (the fused small-level kernel sets bits in an output bitmap)
             if (bottomUp) {
                 word = c0 == ws ? firstFout : fout[w];
+                if (b == 0) fin[w] = 0ull;
             } else if (b == 0) {
                 if (word) a.visited[w] |= word;
                 fout[w] = word;
+                fin[w] = 0ull;
                 sWord[t / 8] = word;
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
@@ -454,7 +793,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
                 const unsigned li = t + i * nb;  // vertex index within the chunk
                 if ((sWord[li >> 6] >> (li & 63)) & 1ull) {
                     const uint32_t v = c0 * 64 + li;
-                    a.levels[v] = int32_t(L);
+                    a.levels[v] = int32_t(Le);
                     if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
                     news += 1;
                 }
@@ -540,6 +879,8 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     a.alpha = g.opts.alpha;
     a.beta = g.opts.beta;
     a.levelFromCtrl = 0;
+    // a trace times every level's own kernels
+    a.fuseQuads = (trace || g.opts.fuse_quads < 0) ? 0u : unsigned(g.opts.fuse_quads);
     dev::BfsArgs ar = a;
     ar.levelFromCtrl = 1;
     const GraphView gv = g.view();
@@ -547,6 +888,9 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     const ExpandKernel expand = expand_kernel_for(g);
     BLEST_HIP(hipEventRecord(q.ev0, q.stream));
     hipLaunchKernelGGL(dev::bfs_init_kernel, grid, block, 0, q.stream, gv, a);
+    // small levels between chunks of level kernels: from the root, and after every chunk
+    const bool fused = a.fuseQuads != 0;
+    if (fused) hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, 1u);
     if (trace) {
         size_t next = 0;
         auto mark = [&] {
@@ -585,7 +929,16 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
             hipLaunchKernelGGL(expand, grid, block, 0, q.stream, gv, ar, k);
             hipLaunchKernelGGL(dev::bfs_update_kernel, grid, block, 0, q.stream, gv, ar, k);
         },
-        launch_key(a.alpha, a.beta, a.direction, g.opts.td_quads | (g.opts.bu_group << 8)));
+        launch_key(a.alpha, a.beta, a.direction,
+                   g.opts.td_quads | (g.opts.bu_group << 8) | (uint64_t(a.fuseQuads) << 16)),
+        fused ? std::function<void(unsigned)>([&](unsigned L) {
+            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, L);
+        })
+              : nullptr,
+        fused ? std::function<void(unsigned)>([&](unsigned k) {
+            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, ar, k);
+        })
+              : nullptr);
     BLEST_HIP(hipGetLastError());
     BLEST_HIP(hipEventRecord(q.ev1, q.stream));
     q.lastSource = src;
@@ -599,7 +952,9 @@ void bfs_finish(Query& q, int32_t* hostLevels, blest_bfs_stats* st) {
     dev::Ctrl c;
     BLEST_HIP(hipMemcpyAsync(&c, q.ctrl, sizeof(c), hipMemcpyDeviceToHost, q.stream));
     BLEST_HIP(hipStreamSynchronize(q.stream));
-    q.levelHint = c.levels;
+    // level kernels the next traversal enqueues before the fused kernel takes over: up to
+    // the final run of small shrinking levels, else all this traversal used
+    q.levelHint = c.tailFrom ? c.tailFrom - 1 : c.levels - c.fusedLevels;
     float ms = 0;
     BLEST_HIP(hipEventElapsedTime(&ms, q.ev0, q.ev1));
     if (st) {

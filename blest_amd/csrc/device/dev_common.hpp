@@ -12,12 +12,17 @@ constexpr int kWave = 64;
 // Direction and running totals of a level-synchronous traversal.  Level L's expand
 // kernel reads slot (L-1)&1 and its block 0 writes slot L&1, so readers and the
 // writer never share a slot within one kernel.
+//
+// A kernel's level argument counts the enqueued levels; the BFS distance it works on is
+// that plus `shift`, the levels the fused small-level kernels ran between enqueued levels
+// (single-source BFS only, 0 elsewhere).  Slots, counters and frontier buffers follow the
+// enqueued level, so the shift travels inside the state the kernels already read.
 struct LevelState {
     unsigned dir;        // 0 top-down, 1 bottom-up (the level that wrote the slot)
     unsigned prevNf;     // frontier size that level consumed
     unsigned visited;    // vertices (or vertex-root pairs) reached before that level
     unsigned td, bu;     // levels run in each direction
-    unsigned pad;
+    unsigned shift;      // distance of the level minus its enqueued level
     unsigned long long cumEdges;  // degree sum of the vertices reached before that level
     unsigned long long buMask;    // bit l-1: level l ran bottom-up (l <= 64)
 };
@@ -33,6 +38,9 @@ struct alignas(256) Ctrl {
     unsigned iterations;
     unsigned levelBase;           // first level of the running replayed chunk (graph mode)
     unsigned histTop;             // multi-source: last level whose frontier words are complete
+    unsigned fusedLevels;         // levels minus the enqueued levels the traversal used
+    unsigned tailFrom;            // single-source: first enqueued level of the final run of
+                                  // small top-down levels with a shrinking frontier (0: none)
     unsigned listCount[3];        // frontier list length (vsets) per slot
     unsigned newCount[3];         // vertices (or vertex-root pairs) discovered per slot
     unsigned long long frontEdges[3];  // degree sum of the discovered vertices per slot

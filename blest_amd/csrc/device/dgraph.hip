@@ -39,16 +39,17 @@ __global__ void advance_levels_kernel(Ctrl* c, unsigned count) {
 }
 }  // namespace dev
 
-uint64_t launch_key(double alpha, double beta, int direction, uint32_t extra) {
+uint64_t launch_key(double alpha, double beta, int direction, uint64_t extra) {
     uint64_t a, b;
     std::memcpy(&a, &alpha, 8);
     std::memcpy(&b, &beta, 8);
     uint64_t h = 0x9E3779B97F4A7C15ull;
-    for (uint64_t x : {a, b, uint64_t(uint32_t(direction)), uint64_t(extra)}) h = (h ^ x) * 0x100000001B3ull + 0x632BE59BD9B4E019ull;
+    for (uint64_t x : {a, b, uint64_t(uint32_t(direction)), extra}) h = (h ^ x) * 0x100000001B3ull + 0x632BE59BD9B4E019ull;
     return h | 1;  // never 0 (0 = not captured)
 }
 
-static void capture_levels(Query& q, LevelGraph& lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
+static void capture_levels(Query& q, LevelGraph& lg, const std::function<void(unsigned)>& levelRel, uint64_t key,
+                           const std::function<void(unsigned)>& afterRel) {
     if (lg.exec) {
         BLEST_HIP(hipGraphExecDestroy(lg.exec));
         lg.exec = nullptr;
@@ -56,6 +57,7 @@ static void capture_levels(Query& q, LevelGraph& lg, const std::function<void(un
     const unsigned C = kGraphChunkLevels;
     BLEST_HIP(hipStreamBeginCapture(q.stream, hipStreamCaptureModeThreadLocal));
     for (unsigned k = 0; k < C; ++k) levelRel(k);
+    if (afterRel) afterRel(C);
     hipLaunchKernelGGL(dev::advance_levels_kernel, dim3(1), dim3(64), 0, q.stream, q.ctrl, C);
     hipGraph_t graph = nullptr;
     const hipError_t launchErr = hipGetLastError();
@@ -92,11 +94,12 @@ static bool poll_done(Query& q, const std::function<void()>& tail) {
 // doubled chunk of levels that only find the done flag (on the 4096-square grid, with
 // depths of 60-80 levels, doubling launched up to 230 levels for a 75-level traversal).
 void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail,
-                LevelGraph* lg, const std::function<void(unsigned)>& levelRel, uint64_t key) {
+                LevelGraph* lg, const std::function<void(unsigned)>& levelRel, uint64_t key,
+                const std::function<void(unsigned)>& after, const std::function<void(unsigned)>& afterRel) {
     const unsigned limit = q.g->n + 2;
     const int mode = q.g->opts.use_graphs;
     if (lg && levelRel && !q.noGraphs && (mode == 1 || (mode == 0 && q.levelHint >= kGraphMinLevels))) {
-        if (!lg->exec || lg->key != key) capture_levels(q, *lg, levelRel, key);
+        if (!lg->exec || lg->key != key) capture_levels(q, *lg, levelRel, key, afterRel);
         const unsigned C = lg->chunk;
         const unsigned maxReps = std::max(1u, 4096u / C);
         unsigned launched = 0;
@@ -111,9 +114,16 @@ void run_levels(Query& q, const std::function<void(unsigned)>& level, const std:
         }
     }
     unsigned L = 1;
-    unsigned chunk = std::max(4u, q.levelHint + 2), more = std::max(4u, q.levelHint / 8);
+    // with `after` (the fused small-level kernel, which finishes a traversal whose last
+    // levels are small): levelHint levels, then `after`; a traversal that needs more
+    // level kernels than the last costs one more poll.  (One spare level and a second
+    // `after` measured slower: an enqueued level that finds the traversal done still
+    // costs ~11 us, and misses were rare.)
+    unsigned chunk = after ? q.levelHint : std::max(4u, q.levelHint + 2);
+    unsigned more = std::max(after ? 1u : 4u, q.levelHint / 8);
     for (;;) {
         for (unsigned i = 0; i < chunk && L <= limit; ++i) level(L++);
+        if (after) after(L);
         if (poll_done(q, tail)) break;
         if (L > limit) throw Error(BLEST_ERR_DEVICE, "traversal did not finish within n + 2 levels");
         chunk = more;
@@ -169,6 +179,7 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->use_graphs = 0;
     o->td_quads = 2;
     o->bu_group = 8;
+    o->fuse_quads = 2048;
 }
 
 }  // extern "C"
@@ -198,6 +209,8 @@ static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* g
         if (opts.td_quads != 1 && opts.td_quads != 2 && opts.td_quads != 4)
             throw Error(BLEST_ERR_INVALID_ARGUMENT, "td_quads must be 1, 2 or 4");
         if (opts.bu_group != 4 && opts.bu_group != 8) throw Error(BLEST_ERR_INVALID_ARGUMENT, "bu_group must be 4 or 8");
+        if (opts.fuse_quads == 0) opts.fuse_quads = 2048;
+        if (opts.fuse_quads > 65536) throw Error(BLEST_ERR_INVALID_ARGUMENT, "fuse_quads must be at most 65536");
         auto* h = new blest_device_graph();
         DeviceGraph& g = h->g;
         try {
@@ -307,6 +320,12 @@ static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* g
                     if (csr->nnz())
                         BLEST_HIP(hipMemcpy(g.csrIdx, csr->rows.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
                     g.hasCsr = true;
+                    // vertices without in-neighbours: a bottom-up sweep skips them
+                    std::vector<unsigned long long> noIn(g.W, 0ull);
+                    for (uint32_t v = 0; v < c.n; ++v)
+                        if (csr->colPtrs[v + 1] == csr->colPtrs[v]) noIn[v >> 6] |= 1ull << (v & 63);
+                    g.noIn = g.alloc<unsigned long long>(g.W);
+                    BLEST_HIP(hipMemcpy(g.noIn, noIn.data(), 8 * size_t(g.W), hipMemcpyHostToDevice));
                 }
             }
             if (inv_perm) {
@@ -383,6 +402,14 @@ blest_status blest_device_graph_set_tiles(blest_device_graph* h, uint32_t td_qua
         if (bu_group != 4 && bu_group != 8) throw Error(BLEST_ERR_INVALID_ARGUMENT, "bu_group must be 4 or 8");
         h->g.opts.td_quads = td_quads;
         h->g.opts.bu_group = bu_group;
+    });
+}
+
+blest_status blest_device_graph_set_fuse(blest_device_graph* h, int32_t fuse_quads) {
+    return guarded([&] {
+        if (!h) throw Error(BLEST_ERR_INVALID_ARGUMENT, "device graph is null");
+        if (fuse_quads > 65536) throw Error(BLEST_ERR_INVALID_ARGUMENT, "fuse_quads must be at most 65536");
+        h->g.opts.fuse_quads = fuse_quads ? fuse_quads : 2048;
     });
 }
 

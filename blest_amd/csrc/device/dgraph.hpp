@@ -43,6 +43,7 @@ struct GraphView {
     const uint32_t* masks;        // Q mask words
     const uint32_t* csrPtr;       // n + 1, bottom-up in-neighbour lists (nullable)
     const uint32_t* csrIdx;       // nnz
+    const unsigned long long* noIn;  // W words: vertices without in-neighbours (nullable)
     unsigned long long nnz;       // directed edges
 };
 
@@ -62,6 +63,7 @@ struct DeviceGraph {
     uint32_t *unitQuad = nullptr, *unitRset = nullptr, *rsetUnit = nullptr, *masks = nullptr;
     uint4* rows = nullptr;
     uint32_t *csrPtr = nullptr, *csrIdx = nullptr;
+    unsigned long long* noIn = nullptr;  // bit v: v has no in-neighbour (never found bottom-up)
     uint32_t *permNewToOld = nullptr, *permOldToNew = nullptr;  // device copies
     std::vector<uint32_t> hostOldToNew, hostNewToOld;
     std::vector<uint32_t> hostDegree;  // per (reordered) vertex, for GTEPS accounting
@@ -81,6 +83,7 @@ struct DeviceGraph {
         g.masks = masks;
         g.csrPtr = csrPtr;
         g.csrIdx = csrIdx;
+        g.noIn = noIn;
         g.nnz = nnz;
         return g;
     }
@@ -170,10 +173,14 @@ struct Query {
 // level levelBase + k), deep traversals replay a captured chunk of levels instead
 // (device opts use_graphs: 0 auto, 1 always, 2 never; queries with noGraphs never).
 dim3 level_grid(const DeviceGraph& g);
+// after / afterRel (nullable): a kernel enqueued after each chunk of levels, given the
+// next level (absolute / relative to the replayed chunk); with it the first chunk is the
+// previous traversal's levelHint levels instead of levelHint + 2.
 void run_levels(Query& q, const std::function<void(unsigned)>& level, const std::function<void()>& tail = nullptr,
                 LevelGraph* lg = nullptr, const std::function<void(unsigned)>& levelRel = nullptr,
-                uint64_t key = 0);
-uint64_t launch_key(double alpha, double beta, int direction, uint32_t extra);
+                uint64_t key = 0, const std::function<void(unsigned)>& after = nullptr,
+                const std::function<void(unsigned)>& afterRel = nullptr);
+uint64_t launch_key(double alpha, double beta, int direction, uint64_t extra);
 uint32_t map_source(const DeviceGraph& g, uint32_t src);
 // Levels of multi-source frontier words kept for the final level expansion: up to 16,
 // within 32 GiB of history per query.

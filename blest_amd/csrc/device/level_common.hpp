@@ -21,7 +21,9 @@ struct DirectionParams {
 // L's direction and returns false when the previous level discovered nothing (then
 // block 0 records the final totals and raises ctrl->done).  Every thread of the block
 // must call it.  When pre is given it also receives the inclusive prefix of the list
-// counts (*total = their sum), and cpre likewise of the chunk counts.
+// counts (*total = their sum), and cpre likewise of the chunk counts.  L is the enqueued
+// level; *effLevel receives the distance it works on (L + the state's shift), *outState
+// the state written for it and *inState the previous level's (every thread gets all).
 //
 // Everything is read as one batch of independent loads (the done flag, the state of
 // level L-1, and each counter row once, thread t taking a contiguous run of segments),
@@ -31,7 +33,8 @@ struct DirectionParams {
 __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint32_t segs, unsigned L,
                                             const DirectionParams& p, bool& bottomUp, unsigned* pre = nullptr,
                                             unsigned* total = nullptr, unsigned* cpre = nullptr,
-                                            unsigned* ctotal = nullptr) {
+                                            unsigned* ctotal = nullptr, unsigned* effLevel = nullptr,
+                                            LevelState* outState = nullptr, LevelState* inState = nullptr) {
     __shared__ unsigned long long s_tmp[16];
     __shared__ unsigned s_tmp32[16];
     const unsigned t = threadIdx.x, nb = blockDim.x;
@@ -39,6 +42,7 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
     const size_t base = size_t(L % 3) * segs;
     const unsigned doneIn = ld_relaxed(&c->done);
     const LevelState in = c->state[(L - 1) & 1];
+    const unsigned Le = L + in.shift;
     unsigned long long nf = 0, mf = 0;
     unsigned mine = 0, cmine = 0;  // this thread's running list / chunk counts
     for (unsigned j0 = 0; j0 < per; j0 += 4) {
@@ -82,7 +86,8 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
     __syncthreads();
     if (nf == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            c->levels = L - 1;
+            c->levels = Le - 1;
+            c->fusedLevels = in.shift;
             c->tdLevels = in.td;
             c->buLevels = in.bu;
             c->visited = in.visited;
@@ -93,7 +98,7 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
         return false;
     }
     const unsigned long long cum = in.cumEdges + mf;
-    bottomUp = in.dir != 0;
+    bottomUp = (in.dir & 1u) != 0;
     if (p.direction == 0 && p.canBottomUp) {
         const double mu = double(p.totalEdges - min(cum, p.totalEdges));
         if (!bottomUp) {
@@ -105,26 +110,29 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
             bottomUp = false;
         }
     } else {
-        bottomUp = (p.direction == 2 && p.canBottomUp && L > 1);
+        bottomUp = (p.direction == 2 && p.canBottomUp && Le > 1);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        LevelState out;
-        out.dir = bottomUp ? 1u : 0u;
-        out.prevNf = unsigned(min(nf, 0xFFFFFFFFull));
-        out.visited = unsigned(min(in.visited + nf, 0xFFFFFFFFull));
-        out.td = in.td + (bottomUp ? 0u : 1u);
-        out.bu = in.bu + (bottomUp ? 1u : 0u);
-        out.pad = 0;
-        out.cumEdges = cum;
-        out.buMask = in.buMask | ((bottomUp && L <= 64) ? (1ull << (L - 1)) : 0ull);
-        c->state[L & 1] = out;
-    }
+    LevelState out;
+    out.dir = bottomUp ? 1u : 0u;
+    out.prevNf = unsigned(min(nf, 0xFFFFFFFFull));
+    out.visited = unsigned(min(in.visited + nf, 0xFFFFFFFFull));
+    out.td = in.td + (bottomUp ? 0u : 1u);
+    out.bu = in.bu + (bottomUp ? 1u : 0u);
+    out.shift = in.shift;
+    out.cumEdges = cum;
+    out.buMask = in.buMask | ((bottomUp && Le <= 64) ? (1ull << (Le - 1)) : 0ull);
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->state[L & 1] = out;
+    if (effLevel) *effLevel = Le;
+    if (outState) *outState = out;
+    if (inState) *inState = in;
     return true;
 }
 
 __device__ __forceinline__ void level_state_reset(Ctrl* c) {
     c->done = 0;
     c->levelBase = 1;
+    c->fusedLevels = 0;
+    c->tailFrom = 0;
     c->state[0] = LevelState{0, 0, 0, 0, 0, 0, 0ull, 0ull};
     c->state[1] = c->state[0];
 }

@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
     if (ld_relaxed(&c->done)) return;
-    const bool bottomUp = c->state[L & 1].dir != 0;
+    const bool bottomUp = (c->state[L & 1].dir & 1u) != 0;
     unsigned long long* fout = mfront<NW>(a, g.n, L);
     const bool writeLevels = L > a.histLevels;
     uint32_t* lout = (L & 1) ? a.list0 : a.list1;

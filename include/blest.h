@@ -207,6 +207,10 @@ typedef struct blest_device_opts {
                                 streams per step, 1, 2 (default) or 4 */
     uint32_t bu_group;       /* bottom-up staging depth: bitmap words a wave probes together, 4
                                 or 8 (default) */
+    int32_t fuse_quads;      /* single-source BFS: a top-down level whose frontier's slices fill
+                                at most this many quads (4 slices) runs inside one workgroup,
+                                and so do the small levels after it, with no kernel boundary
+                                between them; 0 = default (2048), < 0 = never, at most 65536 */
 } blest_device_opts;
 void blest_device_opts_default(blest_device_opts* o);
 
@@ -242,6 +246,8 @@ blest_status blest_device_graph_set_thresholds(blest_device_graph* dg, double al
 /* Change the tile knobs (td_quads, bu_group) of a device graph; same rules as
  * blest_device_graph_set_thresholds. */
 blest_status blest_device_graph_set_tiles(blest_device_graph* dg, uint32_t td_quads, uint32_t bu_group);
+/* Change fuse_quads of a device graph (same rules as in blest_device_opts). */
+blest_status blest_device_graph_set_fuse(blest_device_graph* dg, int32_t fuse_quads);
 
 /* Autotuner (no reference counterpart; the reference hard-codes its launch shape,
  * BFS/BVSSBFSKernel.cuh:1311-1358): coordinate descent over blocks_per_cu (1, 2, 4, 8),

@@ -109,3 +109,49 @@ def test_tile_knobs_rejected(blest):
     lib = blest._lib.load()
     assert lib.blest_device_graph_set_tiles(dg._h, 8, 8) == 1
     assert lib.blest_device_graph_set_tiles(dg._h, 2, 5) == 1
+
+
+FUSE_GRAPHS = [
+    ("rmat13", lambda B: B.CSC.rmat(13, 8, seed=21), "JACCARD"),
+    ("grid_sparse", lambda B: B.CSC.grid(72, 0.62, 0.0, 9), None),  # many components, thin paths
+    ("grid_deep", lambda B: B.CSC.grid(120, 1.0, 0.0, 4), "RCM"),    # ~240 levels, all small
+]
+
+
+@pytest.mark.parametrize("name,make,reorder", FUSE_GRAPHS, ids=[f[0] for f in FUSE_GRAPHS])
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_fused_small_levels(blest, oracle, name, make, reorder, direction):
+    """Small levels run fused in one workgroup (at the default bound, at the largest one,
+    at a tiny one that hands back after every first level, and never): every setting
+    gives the oracle's levels and the same level count, reach and traversed edges."""
+    print(f"case fused levels {name} direction={direction}", flush=True)
+    g = make(blest)
+    cp, rows = g.arrays()
+    inv = g.reorder(getattr(blest.Reorder, reorder), window=4096) if reorder else None
+    b = blest.BVSS(8, 4).construct_from_csc(g)
+    dg = blest.DeviceGraph(b, g, inv, direction=direction)
+    k = blest.BVSSBFSKernel(dg)
+    roots = g.pick_roots(3, seed=5)
+    if inv is not None:
+        roots = np.argsort(inv)[roots]
+    for r in [int(x) for x in roots]:
+        want = oracle.bfs(g.n, cp, rows, r)
+        got = {}
+        for fuse in (-1, 0, 32, 65536):
+            dg.set_fuse(fuse)
+            for rep in range(2):  # the second run starts from the first one's level hint
+                res = k.run(r)
+                np.testing.assert_array_equal(res.levels, want, err_msg=f"fuse={fuse} rep={rep}")
+            got[fuse] = (res.total_levels, res.no_visited, res.edges_in_component)
+        assert len(set(got.values())) == 1, got
+        assert got[-1][0] == want.max() + 1
+
+
+def test_set_fuse_rejected(blest):
+    g = blest.CSC.rmat(10, 8, seed=8)
+    b = blest.BVSS(8, 4).construct_from_csc(g)
+    with pytest.raises(blest.BlestError):
+        blest.DeviceGraph(b, g, None, fuse_quads=65537)
+    dg = blest.DeviceGraph(b, g, None)
+    with pytest.raises(blest.BlestError):
+        dg.set_fuse(1 << 20)
