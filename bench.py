@@ -494,14 +494,15 @@ def main():
         opts["td_quads"] = a.td_quads
     if a.bu_group:
         opts["bu_group"] = a.bu_group
-    if a.autotune and not (a.blocks_per_cu or a.alpha or a.beta or a.td_quads or a.bu_group):
+    if a.autotune and not (a.blocks_per_cu or a.alpha or a.beta or a.td_quads or a.bu_group or a.fuse_quads):
         # launch shape and switch thresholds chosen by the native autotuner on some of the roots
         t0 = time.time()
         best, best_ms, trials = B.autotune_bfs(prep.bvss, prep.csc, prep.inv_perm, prep.roots[:a.autotune_roots],
                                                **opts)
-        for k in ("blocks_per_cu", "td_quads", "bu_group", "alpha", "beta"):
+        for k in ("blocks_per_cu", "td_quads", "bu_group", "alpha", "beta", "fuse_quads"):
             opts[k] = best[k]
         tuned = {"blocks_per_cu": best["blocks_per_cu"], "td_quads": best["td_quads"], "bu_group": best["bu_group"],
+                 "fuse_quads": best["fuse_quads"],
                  "alpha": best["alpha"], "beta": best["beta"], "trials": trials, "tuning_ms_per_bfs": round(best_ms, 4)}
         log(f"[bench] autotuned in {time.time() - t0:.1f}s: {tuned}")
     t0 = time.time()

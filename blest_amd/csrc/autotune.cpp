@@ -57,6 +57,7 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
         blest_device_opts_default(&o);
         if (base) o = *base;
         if (!g) o.keep_csr = 0;
+        if (o.fuse_quads == 0) o.fuse_quads = 2048;
         uint32_t runs = 0;
         constexpr double kMargin = 0.99;  // a new best must be > 1% faster
         double bestMs = std::numeric_limits<double>::infinity();
@@ -113,6 +114,19 @@ extern "C" blest_status blest_autotune_bfs(const blest_bvss* b, const blest_csc*
                 if (ms < bestMs * kMargin) {
                     bestMs = ms;
                     bestOpts.beta = be;
+                }
+            }
+        }
+        // the fused small-level bound (a caller's "never" stays never)
+        if (keep.dg && bestOpts.fuse_quads > 0) {
+            for (int32_t fq : {1024, 2048, 4096, 512}) {
+                if (fq == bestOpts.fuse_quads) continue;
+                check(blest_device_graph_set_fuse(keep.dg, fq));
+                const double ms = time_roots(keep.q, roots, nroots);
+                ++runs;
+                if (ms < bestMs * kMargin) {
+                    bestMs = ms;
+                    bestOpts.fuse_quads = fq;
                 }
             }
         }

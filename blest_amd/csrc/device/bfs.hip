@@ -89,7 +89,6 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
         const unsigned long long x = (w == sw) ? sb : 0ull;
         a.visited[w] = x;
         a.front1[w] = x;
-        a.front0[w] = 0ull;  // a level's output bitmap starts empty
     }
     unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
     for (size_t i = tid; i < size_t(g.W) * 8; i += nthreads) nb64[i] = 0ull;
@@ -130,29 +129,28 @@ __global__ void __launch_bounds__(kProducerBlock) bfs_init_kernel(GraphView g, B
 // in each kernel, for a few hundred slices.  bfs_fused_kernel, one workgroup, runs such
 // levels back to back with no kernel boundary between them.  It is enqueued after the init
 // kernel and after each chunk of level kernels; kernel argument Lk is the enqueued level
-// whose inputs it reads (lists, counters, input bitmap, state slot), and when it stops it
-// leaves the inputs of that same enqueued level, Lk's state carrying the levels it ran as
-// its shift, so the expand kernel enqueued as Lk goes on from there.  Per level:
-//   queue   the frontier's rsets, in the top-down discovery bytes `nextB` (all zero
-//           between levels; two queues of up to R entries, zeroed again after use): the
-//           first level's from its lists (the first vset of each listed rset), later
-//           ones from the discoveries.  A level runs here while its queue holds at most
-//           kFuseTile rsets and fuseQuads quads (an rset's quads are contiguous);
+// whose inputs it reads (lists, counters, frontier bitmap, state slot), and when it stops
+// it leaves the inputs of that same enqueued level, Lk's state carrying the levels it ran
+// as its shift, so the expand kernel enqueued as Lk goes on from there.  Per level:
+//   queue   the frontier's rsets (the first level's from its lists: the first vset of each
+//           listed rset, or the root's rset; later ones from the discoveries), their quad
+//           and vset ranges and frontier bytes in LDS.  A level runs here while its queue
+//           holds at most kFuseTile rsets and fuseQuads quads (an rset's quads are
+//           contiguous);
+//   clear   the frontier words of the queue's rsets (their bytes are in LDS now), so the
+//           same bitmap receives the level's discoveries;
 //   stream  the queue's quads, kFuseQpl per thread per pass, the owner entry found by
-//           binary search in an LDS prefix of the entries' quad counts; a slice hit on a
+//           binary search in the LDS prefix of the entries' quad counts; a slice hit on a
 //           row not yet visited claims the row with a 32-bit atomicOr on `visited`; the
-//           winner writes the level, sets the row's bit in the output bitmap, and when
-//           that atomic's old value shows no bit of the row's rset yet, appends the rset;
-//   clear   the input bitmap's words of the queue's rsets, so the buffer is empty again
-//           when it is the output bitmap two levels on.
-// A level's output bitmap starts empty: the init kernel clears it for level 1 and every
-// update kernel clears the input bitmap it consumed.  A traversal that ends here raises
-// the done flag as level_begin does.  Otherwise, when the next level is too big, bottom-up
-// by the direction rule, or the schedule is bottom-up after level 1, the workgroup writes
-// the next level's lists (in the update kernel's form, from the start of the list arrays:
-// one flat range that segment 0's counters describe) and moves its frontier bits into the
-// input bitmap.  Bits of rsets without vsets stay in the first input bitmap (no list names
-// them): their vertices have no out-edges, so no direction ever tests them.
+//           winner writes the level, sets the row's bit in the bitmap, and when that
+//           atomic's old value shows no bit of the row's rset yet, appends the rset.
+// A traversal that ends here raises the done flag as level_begin does.  Otherwise, when
+// the next level is too big, bottom-up by the direction rule, or the schedule is bottom-up
+// after level 1, the workgroup writes that level's lists (in the update kernel's form,
+// from the start of the list arrays: one flat range that segment 0's counters describe);
+// its frontier is already in the bitmap the expand kernel reads.  Bits of rsets without
+// vsets may stay in that bitmap (no list names them, so no queue clears them): their
+// vertices have no out-edges, so no direction ever tests them.
 constexpr int kFusedBlock = 1024;
 constexpr unsigned kFuseTile = 2 * kFusedBlock;  // rsets a fused level's queue may hold
 constexpr int kFuseQpl = 4;                      // quads per thread per pass
@@ -165,7 +163,8 @@ typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 // A queue of rsets: entries below kFuseTile in LDS (the level prefix arrays, free once the
 // first level's lists are read), later ones in global memory at the same index (the
 // top-down discovery bytes `nextB`, all zero between levels: two queues of up to R
-// entries fit in its 64 W bytes; an entry used there is zeroed again after use).
+// entries fit in its 64 W bytes; an entry used there is zeroed again after use).  Only
+// the queue of a level too big to run here reaches past kFuseTile.
 struct FusedQueue {
     uint32_t* lds;
     uint32_t* glb;
@@ -182,25 +181,33 @@ struct FusedQueue {
 };
 static_assert(kFuseTile <= unsigned(kMaxSegs), "an LDS queue lives in a level prefix array");
 
+// Per-entry LDS arrays of a queue of at most kFuseTile rsets.
+struct FusedInfo {
+    uint32_t* pf;   // exclusive prefix of the entries' quad counts
+    uint32_t* q0;   // first quad
+    uint32_t* fbq;  // frontier byte, replicated to the 4 slots of a quad
+    uint32_t* u0;   // first vset
+    uint32_t* uc;   // vsets
+};
+
 // Listed vsets (<= 32 quads each) with which a level enters the fused kernel: the entering
 // level never exceeds fuseQuads, and holds at most one list entry per thread.
 __host__ __device__ constexpr unsigned fuse_vsets(unsigned fuseQuads) {
     return fuseQuads / 32 < unsigned(kFusedBlock) ? fuseQuads / 32 : unsigned(kFusedBlock);
 }
 
-// Entries 2t, 2t + 1 of queue q (nq <= kFuseTile entries): exclusive quad prefix into pf,
-// first quad into q0, frontier byte (replicated to the 4 slots of a quad) into fbq;
-// returns the queue's quad total.  Every thread must call it.
-__device__ __forceinline__ unsigned fused_queue_info(const GraphView& g, const FusedQueue& q, unsigned nq,
-                                                     const unsigned long long* front, uint32_t* pf, uint32_t* q0,
-                                                     uint32_t* fbq, unsigned* tmp32) {
+// The info of queue entries 2t, 2t + 1 (nq <= kFuseTile); returns the queue's quad total.
+// Every thread must call it.
+__device__ __forceinline__ unsigned fused_queue_info(const GraphView& g, const uint32_t* q, unsigned nq,
+                                                     const unsigned long long* front, const FusedInfo& fi,
+                                                     unsigned* tmp32) {
     const unsigned t = threadIdx.x;
     uint32_t u0[2], u1[2], fb[2], qs[2], cnt[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const unsigned i = 2 * t + k;
         const bool ok = i < nq;
-        const uint32_t r = ok ? q.lds[i] : 0u;
+        const uint32_t r = ok ? q[i] : 0u;
         u0[k] = ok ? g.rsetUnit[r] : 0u;
         u1[k] = ok ? g.rsetUnit[r + 1] : 0u;
         fb[k] = ok ? front_byte(front, r) : 0u;
@@ -213,30 +220,41 @@ __device__ __forceinline__ unsigned fused_queue_info(const GraphView& g, const F
     }
     unsigned total;
     const unsigned ex = block_exclusive_scan(cnt[0] + cnt[1], tmp32, total);
-    pf[2 * t] = ex;
-    pf[2 * t + 1] = ex + cnt[0];
-    q0[2 * t] = qs[0];
-    q0[2 * t + 1] = qs[1];
-    fbq[2 * t] = fb[0] * 0x01010101u;
-    fbq[2 * t + 1] = fb[1] * 0x01010101u;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const unsigned i = 2 * t + k;
+        fi.pf[i] = k ? ex + cnt[0] : ex;
+        fi.q0[i] = qs[k];
+        fi.fbq[i] = fb[k] * 0x01010101u;
+        fi.u0[i] = u0[k];
+        fi.uc[i] = u1[k] - u0[k];
+    }
     __syncthreads();
     return __builtin_amdgcn_readfirstlane(total);
 }
 
 // Lists of queue entries [0, nq) (2 per thread per step), in the order and form the
 // update kernel gives them (an rset's vsets, or its chunks beyond kChunkVsets vsets),
-// appended at out + ltot / cout + ctot; ltot / ctot advance.  Every thread must call it.
-__device__ __forceinline__ void fused_write_lists(const GraphView& g, const FusedQueue& q, unsigned nq, uint32_t* out,
-                                                  uint2* cout, unsigned* tmp32, unsigned& ltot, unsigned& ctot) {
+// from the start of out / cout; vset ranges from the info when the queue had it
+// (nq <= kFuseTile), else loaded.  Every thread must call it.
+__device__ __forceinline__ void fused_write_lists(const GraphView& g, const FusedQueue& q, unsigned nq,
+                                                  const FusedInfo* fi, uint32_t* out, uint2* cout, unsigned* tmp32,
+                                                  unsigned& ltot, unsigned& ctot) {
     const unsigned t = threadIdx.x, nb = blockDim.x;
+    ltot = ctot = 0;
     for (unsigned base = 0; base < nq; base += 2 * nb) {
         uint32_t u0[2], cnt[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const unsigned i = base + 2 * t + k;
-            const uint32_t r = i < nq ? q.get(i) : 0u;
-            u0[k] = i < nq ? g.rsetUnit[r] : 0u;
-            cnt[k] = i < nq ? g.rsetUnit[r + 1] - u0[k] : 0u;
+            if (fi) {
+                u0[k] = i < nq ? fi->u0[i] : 0u;
+                cnt[k] = i < nq ? fi->uc[i] : 0u;
+            } else {
+                const uint32_t r = i < nq ? q.get(i) : 0u;
+                u0[k] = i < nq ? g.rsetUnit[r] : 0u;
+                cnt[k] = i < nq ? g.rsetUnit[r + 1] - u0[k] : 0u;
+            }
         }
         const uint32_t ch0 = rset_chunks(cnt[0]), ch1 = rset_chunks(cnt[1]);
         const uint32_t l0 = ch0 ? 0u : cnt[0], l1 = ch1 ? 0u : cnt[1];
@@ -254,10 +272,12 @@ __device__ __forceinline__ void fused_write_lists(const GraphView& g, const Fuse
     }
 }
 
-__global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, BfsArgs a, unsigned Lk) {
+// head != 0: launched right after the init kernel, for level 1 (its inputs are the root's
+// rset and the reset state, so level_begin's counter reads are skipped).
+__global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, BfsArgs a, unsigned Lk, int head) {
     const unsigned Lin = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned pre[kMaxSegs], cpre[kMaxSegs];
-    __shared__ uint32_t pf[kFuseTile], q0[kFuseTile], fbq[kFuseTile];
+    __shared__ uint32_t sPf[kFuseTile], sQ0[kFuseTile], sFb[kFuseTile], sU0[kFuseTile], sUc[kFuseTile];
     __shared__ unsigned cntr, tmp32[16];
     __shared__ unsigned long long tmp64[16];
     Ctrl* c = a.ctrl;
@@ -273,43 +293,64 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
     bool bottomUp = false;
     unsigned total = 0, ctotal = 0, Le = 0;
     LevelState cur, in;  // state of the level about to be expanded, and of the one before
-    if (!level_begin(c, a.sc, segs, Lin, dp, bottomUp, pre, &total, cpre, &ctotal, &Le, &cur, &in)) return;
-    // too big or bottom-up: the expand kernel enqueued as Lin takes it (its state slot
-    // written above is rewritten there from the same inputs)
-    const unsigned scale = cur.prevNf < in.prevNf ? kFuseShrinkScale : 1u;
-    if (bottomUp || total + ctotal * kChunkVsets > fuse_vsets(scale * a.fuseQuads)) return;
-    if (t == 0 && !c->tailFrom && Lin > 1) c->tailFrom = Lin;
+    if (head) {
+        // level 1: frontier {src}; level_begin's state, direction rule and list size
+        const uint32_t r = a.src >> 3;
+        const uint32_t u0 = g.rsetUnit[r], u1 = g.rsetUnit[r + 1];
+        const unsigned long long deg = g.csrPtr ? g.csrPtr[a.src + 1] - g.csrPtr[a.src] : 0ull;
+        const uint32_t ch = rset_chunks(u1 - u0);
+        total = ch ? 0u : u1 - u0;
+        ctotal = ch;
+        Le = 1;
+        cur = LevelState{0, 1, 1, 1, 0, 0, deg, 0ull};
+        bottomUp = a.direction == 0 && dp.canBottomUp &&
+                   double(deg) * a.alpha > double(g.nnz - min(deg, g.nnz));
+        if (bottomUp || total + ctotal * kChunkVsets > fuse_vsets(a.fuseQuads)) return;
+    } else {
+        if (!level_begin(c, a.sc, segs, Lin, dp, bottomUp, pre, &total, cpre, &ctotal, &Le, &cur, &in)) return;
+        // too big or bottom-up: the expand kernel enqueued as Lin takes it (its state slot
+        // written above is rewritten there from the same inputs)
+        const unsigned scale = cur.prevNf < in.prevNf ? kFuseShrinkScale : 1u;
+        if (bottomUp || total + ctotal * kChunkVsets > fuse_vsets(scale * a.fuseQuads)) return;
+        if (t == 0 && !c->tailFrom) c->tailFrom = Lin;
+    }
     uint32_t* lists = (Lin & 1) ? a.list1 : a.list0;  // read first, rewritten when handing back
     uint2* chunks = (Lin & 1) ? a.chunk1 : a.chunk0;
-    unsigned long long* fin = (Lin & 1) ? a.front1 : a.front0;  // the frontier ends here
-    // the first level's rsets: list entry t and chunk entry t (at most one each)
-    uint32_t cand0 = kNoRset, cand1 = kNoRset;
-    if (t < total) {
-        const uint32_t v = list_entry(lists, a.sv, pre, t);
-        const uint32_t r = g.unitRset[v], rp = v ? g.unitRset[v - 1] : kNoRset;
-        if (rp != r) cand0 = r;
-    }
-    if (t < ctotal) {
-        const uint2 ch = chunk_entry(chunks, a.sv, cpre, t);
-        const uint32_t r = g.unitRset[ch.x], rp = ch.x ? g.unitRset[ch.x - 1] : kNoRset;
-        if (rp != r) cand1 = r;
-    }
+    unsigned long long* F = (Lin & 1) ? a.front1 : a.front0;  // frontier bitmap, in and out
+    const FusedInfo fi{sPf, sQ0, sFb, sU0, sUc};
     FusedQueue qc{pre, reinterpret_cast<uint32_t*>(a.nextB)};
     FusedQueue qn{cpre, reinterpret_cast<uint32_t*>(a.nextB) + g.R};
-    if (t == 0) cntr = 0;
-    __syncthreads();  // the list prefixes are read: they become the queues
-    if (cand0 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand0;  // <= 1024 + 64 entries
-    if (cand1 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand1;
-    __syncthreads();
-    unsigned nq = __builtin_amdgcn_readfirstlane(cntr);
-    unsigned long long* X = fin;                                  // input bitmap
-    unsigned long long* Y = (Lin & 1) ? a.front0 : a.front1;      // output bitmap (empty)
+    unsigned nq;
+    if (head) {
+        if (t == 0) pre[0] = a.src >> 3;  // read below by this thread only (entry 0)
+        nq = 1;
+    } else {
+        // list entry t and chunk entry t (at most one each): the first vset of an rset
+        uint32_t cand0 = kNoRset, cand1 = kNoRset;
+        if (t < total) {
+            const uint32_t v = list_entry(lists, a.sv, pre, t);
+            const uint32_t r = g.unitRset[v], rp = v ? g.unitRset[v - 1] : kNoRset;
+            if (rp != r) cand0 = r;
+        }
+        if (t < ctotal) {
+            const uint2 ch = chunk_entry(chunks, a.sv, cpre, t);
+            const uint32_t r = g.unitRset[ch.x], rp = ch.x ? g.unitRset[ch.x - 1] : kNoRset;
+            if (rp != r) cand1 = r;
+        }
+        if (t == 0) cntr = 0;
+        __syncthreads();  // the list prefixes are read: they become the queues
+        if (cand0 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand0;  // <= 1024 + 64 entries
+        if (cand1 != kNoRset) qc.lds[atomicAdd(&cntr, 1u)] = cand1;
+        __syncthreads();
+        nq = __builtin_amdgcn_readfirstlane(cntr);
+    }
     uint32_t* vis32 = reinterpret_cast<uint32_t*>(a.visited);
+    uint32_t* f32 = reinterpret_cast<uint32_t*>(F);
     unsigned Lcur = Le;
     LevelState prev = cur;  // state of level Lcur - 1 (from the second level on)
     unsigned long long prevEdges = 0;  // degree sum of level Lcur's frontier
     for (unsigned it = 0;; ++it) {
-        const unsigned T = nq <= kFuseTile ? fused_queue_info(g, qc, nq, X, pf, q0, fbq, tmp32) : 0xFFFFFFFFu;
+        const unsigned T = nq <= kFuseTile ? fused_queue_info(g, qc.lds, nq, F, fi, tmp32) : 0xFFFFFFFFu;
         if (it > 0) {
             // the entering level always runs here; a later one only while small and top-down
             const unsigned cap = cur.prevNf < prev.prevNf ? kFuseShrinkScale * a.fuseQuads : a.fuseQuads;
@@ -319,16 +360,11 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
                 stop = double(prevEdges) * a.alpha > mu && cur.prevNf > prev.prevNf;
             }
             if (stop) {
-                // level Lcur for the expand kernel enqueued as Lin: its lists, counters,
-                // frontier bitmap and the state of level Lcur - 1 with the shift
-                unsigned ltot = 0, ctot = 0;
-                fused_write_lists(g, qc, nq, lists, chunks, tmp32, ltot, ctot);
-                if (X != fin) {
-                    for (unsigned i = t; i < nq; i += nb) fin[qc.get(i) >> 3] = X[qc.get(i) >> 3];
-                    __syncthreads();  // a word shared by two entries is copied before either clears it
-                    for (unsigned i = t; i < nq; i += nb) X[qc.get(i) >> 3] = 0ull;
-                }
-                for (unsigned i = t; i < nq; i += nb) qc.release(i);
+                // level Lcur for the expand kernel enqueued as Lin: its lists and counters
+                // and the state of level Lcur - 1 with the shift (the frontier is in F)
+                unsigned ltot, ctot;
+                fused_write_lists(g, qc, nq, nq <= kFuseTile ? &fi : nullptr, lists, chunks, tmp32, ltot, ctot);
+                for (unsigned i = kFuseTile + t; i < nq; i += nb) qc.release(i);
                 if (t == 0) {
                     LevelState o = prev;
                     o.shift = Lcur - Lin;
@@ -344,11 +380,12 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
                 return;
             }
         }
-        if (t == 0) cntr = 0;  // every read of it came before the info's barriers
+        // the queue's frontier bytes are in LDS: its words take this level's discoveries
+        for (unsigned i = t; i < nq; i += nb) F[qc.lds[i] >> 3] = 0ull;
+        if (t == 0) cntr = 0;
         __syncthreads();
         unsigned news = 0;
         unsigned long long edges = 0;
-        uint32_t* y32 = reinterpret_cast<uint32_t*>(Y);
         for (unsigned t0 = 0; t0 < T; t0 += nb * kFuseQpl) {
             uint32_t row[4 * kFuseQpl], w[4 * kFuseQpl];
             unsigned hit = 0;  // bit s: slice s hits the frontier
@@ -361,10 +398,10 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
                     unsigned o = 0;  // the last entry whose prefix is <= tt
 #pragma unroll
                     for (unsigned step = kFuseTile / 2; step > 0; step >>= 1)
-                        if (o + step < nq && pf[o + step] <= tt) o += step;
-                    const uint32_t q = q0[o] + (tt - pf[o]);
+                        if (o + step < nq && sPf[o + step] <= tt) o += step;
+                    const uint32_t q = sQ0[o] + (tt - sPf[o]);
                     r4 = stream_load_quad(g.rows, q);
-                    m = stream_load_u32(g.masks, q) & fbq[o];
+                    m = stream_load_u32(g.masks, q) & sFb[o];
                 }
                 row[4 * h + 0] = r4.x;
                 row[4 * h + 1] = r4.y;
@@ -391,7 +428,7 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
             for (int s = 0; s < 4 * kFuseQpl; ++s) {
                 if ((hit >> s) & 1u) {
                     a.levels[row[s]] = int32_t(Lcur);
-                    w[s] = atomicOr(&y32[row[s] >> 5], 1u << (row[s] & 31));
+                    w[s] = atomicOr(&f32[row[s] >> 5], 1u << (row[s] & 31));
                     if (g.csrPtr) {  // both list bounds in one 8-byte load
                         const u32x2_a4 d = *reinterpret_cast<const u32x2_a4*>(g.csrPtr + row[s]);
                         edges += d.y - d.x;
@@ -409,10 +446,6 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
         const unsigned long long mf = (unsigned long long)__builtin_amdgcn_readfirstlane(unsigned(eSum >> 32)) << 32 |
                                       __builtin_amdgcn_readfirstlane(unsigned(eSum));
         const unsigned nNew = __builtin_amdgcn_readfirstlane(cntr);
-        // this level's frontier words and queue entries are no longer read
-        for (unsigned i = t; i < nq; i += nb) {  // nq <= kFuseTile: all in LDS
-            X[qc.lds[i] >> 3] = 0ull;
-        }
         if (nf == 0) {
             if (t == 0) {
                 c->levels = Lcur;
@@ -438,10 +471,6 @@ __global__ void __launch_bounds__(kFusedBlock) bfs_fused_kernel(GraphView g, Bfs
         qc = qn;
         qn = tq;
         nq = nNew;
-        unsigned long long* tb = X;
-        X = Y;
-        Y = tb;
-        __syncthreads();  // the cleared words and queue entries precede the next level's use
     }
 }
 
@@ -717,7 +746,6 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     __shared__ uint32_t sStart[kProducerBlock];
     __shared__ unsigned long long sWord[kProducerBlock / 8];
     Ctrl* c = a.ctrl;
-    unsigned long long* fin = (L & 1) ? a.front1 : a.front0;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     const uint32_t segs = a.sv.segs;
     const unsigned sOut = (L + 1) % 3;
@@ -759,15 +787,11 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             if (raw) nb64[size_t(w) * 8 + b] = 0ull;
         }
         if (w < we) {
-            // the input bitmap is consumed: emptied for the level after next, whose output
-            // it is (the fused small-level kernel sets bits in an output bitmap)
             if (bottomUp) {
                 word = c0 == ws ? firstFout : fout[w];
-                if (b == 0) fin[w] = 0ull;
             } else if (b == 0) {
                 if (word) a.visited[w] |= word;
                 fout[w] = word;
-                fin[w] = 0ull;
                 sWord[t / 8] = word;
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
@@ -890,7 +914,7 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
     hipLaunchKernelGGL(dev::bfs_init_kernel, grid, block, 0, q.stream, gv, a);
     // small levels between chunks of level kernels: from the root, and after every chunk
     const bool fused = a.fuseQuads != 0;
-    if (fused) hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, 1u);
+    if (fused) hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, 1u, 1);
     if (trace) {
         size_t next = 0;
         auto mark = [&] {
@@ -932,11 +956,11 @@ void bfs_launch(Query& q, uint32_t src, bool trace) {
         launch_key(a.alpha, a.beta, a.direction,
                    g.opts.td_quads | (g.opts.bu_group << 8) | (uint64_t(a.fuseQuads) << 16)),
         fused ? std::function<void(unsigned)>([&](unsigned L) {
-            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, L);
+            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, a, L, 0);
         })
               : nullptr,
         fused ? std::function<void(unsigned)>([&](unsigned k) {
-            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, ar, k);
+            hipLaunchKernelGGL(dev::bfs_fused_kernel, dim3(1), dim3(dev::kFusedBlock), 0, q.stream, gv, ar, k, 0);
         })
               : nullptr);
     BLEST_HIP(hipGetLastError());

@@ -251,7 +251,8 @@ blest_status blest_device_graph_set_fuse(blest_device_graph* dg, int32_t fuse_qu
 
 /* Autotuner (no reference counterpart; the reference hard-codes its launch shape,
  * BFS/BVSSBFSKernel.cuh:1311-1358): coordinate descent over blocks_per_cu (1, 2, 4, 8),
- * then td_quads (1, 2, 4), bu_group (4, 8), alpha, then beta, each
+ * then td_quads (1, 2, 4), bu_group (4, 8), alpha, beta, then fuse_quads (512 to 4096;
+ * a base fuse_quads < 0 stays off), each
  * candidate timed as the mean HIP-event time of single-source BFS from `roots`
  * (original IDs) after one warm-up pass; a candidate replaces the best so far only when
  * it is more than 1% faster, so ties between candidates within timing noise keep the

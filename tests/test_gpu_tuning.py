@@ -21,8 +21,9 @@ def test_autotune_then_parity(blest, oracle):
     assert best["alpha"] in (4, 8, 15, 30, 60, 120, 250, 500, 1000)
     assert best["beta"] in (6, 12, 18, 24, 48, 96, 200, 400, 800)
     assert best["td_quads"] in (1, 2, 4) and best["bu_group"] in (4, 8)
+    assert best["fuse_quads"] in (512, 1024, 2048, 4096)
     dg = blest.DeviceGraph(b, g, inv, **{k: best[k] for k in ("blocks_per_cu", "td_quads", "bu_group", "alpha",
-                                                                "beta")})
+                                                                "beta", "fuse_quads")})
     k = blest.BVSSBFSKernel(dg)
     for r in roots_orig:
         np.testing.assert_array_equal(k.run(int(r)).levels, oracle.bfs(g.n, cp, rows, int(r)))
