@@ -420,6 +420,8 @@ def main():
     ap.add_argument("--direction", type=int, default=-1, help="0 optimising, 1 top-down only, 2 bottom-up")
     ap.add_argument("--td-quads", type=int, default=0, help="top-down slice tile: quads per lane per step (1, 2, 4)")
     ap.add_argument("--bu-group", type=int, default=0, help="bottom-up staging depth: words probed together (4, 8)")
+    ap.add_argument("--bu-probe", type=int, default=0,
+                    help="bottom-up first probe: highest-degree in-neighbour (0 default) or list order (< 0)")
     ap.add_argument("--fuse-quads", type=int, default=0,
                     help="quads a small level may stream fused in one workgroup (0 default, < 0 never)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
@@ -486,6 +488,8 @@ def main():
         opts["use_graphs"] = a.use_graphs
     if a.fuse_quads:
         opts["fuse_quads"] = a.fuse_quads
+    if a.bu_probe:
+        opts["bu_probe"] = a.bu_probe
     if a.mbfs:
         print(json.dumps(mbfs_bench(a, prep, opts, algos=tuple(a.mbfs_algo))), flush=True)
         return
@@ -584,13 +588,16 @@ def main():
                                     "ms_per_launch": round(by_dir[d][1] * 1e3 / len(replay_roots), 4)}
         # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
         # summary of this workload, used only when it was taken at the schedule this run
-        # uses (launch shape and switch thresholds); otherwise null
+        # uses (launch shape, switch thresholds, fused small-level bound); otherwise null
         traffic = None
         tf = ROOT / "profiles" / f"bfs_kernel_{a.workload}_traffic.json"
         if tf.exists():
             tj = json.loads(tf.read_text())
             mine = {k: float(opts.get(k, dflt)) for k, dflt in (("blocks_per_cu", 4), ("td_quads", 2), ("bu_group", 8),
-                                                                 ("alpha", 15.0), ("beta", 18.0))}
+                                                                 ("alpha", 15.0), ("beta", 18.0),
+                                                                 ("fuse_quads", 2048))}
+            if mine["fuse_quads"] == 0:
+                mine["fuse_quads"] = 2048.0
             for entry in tj.get("schedules", []):
                 sched = entry.get("schedule") or {}
                 if all(abs(float(sched.get(k, -1)) - v) < 1e-9 for k, v in mine.items()):

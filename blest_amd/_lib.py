@@ -42,7 +42,7 @@ class DeviceOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("block_size", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("alpha", C.c_double), ("beta", C.c_double), ("direction", C.c_int), ("keep_csr", C.c_int),
                 ("timeout_ms", C.c_uint32), ("use_graphs", C.c_int), ("td_quads", C.c_uint32),
-                ("bu_group", C.c_uint32), ("fuse_quads", C.c_int32)]
+                ("bu_group", C.c_uint32), ("fuse_quads", C.c_int32), ("bu_probe", C.c_int32)]
 
 
 class DeviceGraphInfo(C.Structure):

@@ -180,6 +180,7 @@ void blest_device_opts_default(blest_device_opts* o) {
     o->td_quads = 2;
     o->bu_group = 8;
     o->fuse_quads = 2048;
+    o->bu_probe = 0;
 }
 
 }  // extern "C"
@@ -321,11 +322,36 @@ static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* g
                         BLEST_HIP(hipMemcpy(g.csrIdx, csr->rows.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
                     g.hasCsr = true;
                     // vertices without in-neighbours: a bottom-up sweep skips them
+                    const std::vector<uint32_t>& cp = csr->colPtrs;
+                    const std::vector<uint32_t>& ci = csr->rows;
                     std::vector<unsigned long long> noIn(g.W, 0ull);
                     for (uint32_t v = 0; v < c.n; ++v)
-                        if (csr->colPtrs[v + 1] == csr->colPtrs[v]) noIn[v >> 6] |= 1ull << (v & 63);
+                        if (cp[v + 1] == cp[v]) noIn[v >> 6] |= 1ull << (v & 63);
                     g.noIn = g.alloc<unsigned long long>(g.W);
                     BLEST_HIP(hipMemcpy(g.noIn, noIn.data(), 8 * size_t(g.W), hipMemcpyHostToDevice));
+                    if (opts.bu_probe >= 0) {
+                        // the in-neighbour each vertex probes first: its highest-degree one
+                        // (the first such in list order).  High-degree vertices are reached
+                        // first, so at a bottom-up level it is the likeliest to be in the
+                        // frontier, and the sweep reads these probes coalesced instead of
+                        // one random index line per vertex.
+                        std::vector<uint32_t> probe(c.n, 0u);
+                        parallel_blocks(c.n, hw_threads(), [&](uint64_t v0, uint64_t v1, unsigned) {
+                            for (uint64_t v = v0; v < v1; ++v) {
+                                uint32_t best = 0, bestDeg = 0;
+                                for (uint32_t p = cp[v]; p < cp[v + 1]; ++p) {
+                                    const uint32_t u = ci[p], d = cp[u + 1] - cp[u];
+                                    if (p == cp[v] || d > bestDeg) {
+                                        best = u;
+                                        bestDeg = d;
+                                    }
+                                }
+                                probe[v] = best;
+                            }
+                        });
+                        g.probe = g.alloc<uint32_t>(c.n);
+                        BLEST_HIP(hipMemcpy(g.probe, probe.data(), 4 * size_t(c.n), hipMemcpyHostToDevice));
+                    }
                 }
             }
             if (inv_perm) {

@@ -44,6 +44,8 @@ struct GraphView {
     const uint32_t* csrPtr;       // n + 1, bottom-up in-neighbour lists (nullable)
     const uint32_t* csrIdx;       // nnz
     const unsigned long long* noIn;  // W words: vertices without in-neighbours (nullable)
+    const uint32_t* probe;        // n: each vertex's in-neighbour a bottom-up sweep tests first
+                                  // (its highest-degree one; nullable)
     unsigned long long nnz;       // directed edges
 };
 
@@ -64,6 +66,7 @@ struct DeviceGraph {
     uint4* rows = nullptr;
     uint32_t *csrPtr = nullptr, *csrIdx = nullptr;
     unsigned long long* noIn = nullptr;  // bit v: v has no in-neighbour (never found bottom-up)
+    uint32_t* probe = nullptr;           // first bottom-up probe per vertex
     uint32_t *permNewToOld = nullptr, *permOldToNew = nullptr;  // device copies
     std::vector<uint32_t> hostOldToNew, hostNewToOld;
     std::vector<uint32_t> hostDegree;  // per (reordered) vertex, for GTEPS accounting
@@ -84,6 +87,7 @@ struct DeviceGraph {
         g.csrPtr = csrPtr;
         g.csrIdx = csrIdx;
         g.noIn = noIn;
+        g.probe = probe;
         g.nnz = nnz;
         return g;
     }

@@ -211,6 +211,9 @@ typedef struct blest_device_opts {
                                 at most this many quads (4 slices) runs inside one workgroup,
                                 and so do the small levels after it, with no kernel boundary
                                 between them; 0 = default (2048), < 0 = never, at most 65536 */
+    int32_t bu_probe;        /* bottom-up: each vertex first tests its highest-degree
+                                in-neighbour (an n-entry array built at creation), then its
+                                list; 0 = default (on), < 0 = off (list order only) */
 } blest_device_opts;
 void blest_device_opts_default(blest_device_opts* o);
 

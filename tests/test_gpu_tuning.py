@@ -156,3 +156,22 @@ def test_set_fuse_rejected(blest):
     dg = blest.DeviceGraph(b, g, None)
     with pytest.raises(blest.BlestError):
         dg.set_fuse(1 << 20)
+
+
+@pytest.mark.parametrize("bu_probe", [0, -1])
+@pytest.mark.parametrize("direction", [0, 2])
+def test_bottom_up_probe(blest, oracle, bu_probe, direction):
+    """Bottom-up sweeps give the oracle's levels whether each vertex first tests its
+    highest-degree in-neighbour or its first list entry, on a symmetric and a directed
+    graph (the directed one's sweeps read the transpose)."""
+    print(f"case bottom-up probe {bu_probe} direction={direction}", flush=True)
+    rng = np.random.default_rng(5)
+    n = 6000
+    src, dst = rng.integers(0, n, 60000), rng.integers(0, n, 60000)
+    for g in (blest.CSC.rmat(13, 16, seed=31), blest.CSC.from_edges(n, src, dst, undirected=False)):
+        cp, rows = g.arrays()
+        b = blest.BVSS(8, 4).construct_from_csc(g)
+        dg = blest.DeviceGraph(b, g, None, direction=direction, bu_probe=bu_probe)
+        k = blest.BVSSBFSKernel(dg)
+        for r in g.pick_roots(3, seed=2):
+            np.testing.assert_array_equal(k.run(int(r)).levels, oracle.bfs(g.n, cp, rows, int(r)))
