@@ -629,14 +629,13 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             if (pe[k] > pb[k]) cand |= 1u << k;
         }
         unsigned found = 0;
-        // The first probe of every open vertex of the group at once (all index loads,
-        // then all frontier words): most vertices of a bottom-up level hit there, so the
-        // group costs one chain of dependent loads, not one per word.  The probe is the
-        // vertex's highest-degree in-neighbour (g.probe, read coalesced: lane l of word w
-        // loads entry 64w + l), else its first list entry.  A vertex still open afterwards
-        // scans its list from the start, or from its second entry when the probe was the
-        // first entry (pb stays the list start: its degree is counted when it is found).
-        const uint32_t skip = g.probe ? 0u : 1u;
+        // The first in-neighbour of every open vertex of the group at once (all index
+        // loads, then all frontier words): most vertices of a bottom-up level hit there,
+        // so the group costs one chain of dependent loads, not one per word.  With the
+        // lists sorted by degree the first entries are also in g.probe, read coalesced
+        // (lane l of word w loads entry 64w + l).  A vertex still open afterwards goes on
+        // from its second entry (pb stays the list start: its degree is counted when it
+        // is found).
         {
             uint32_t nb0[kBuGroup], fw0[kBuGroup];
 #pragma unroll
@@ -654,7 +653,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             }
         }
         int k = cand ? __builtin_ctz(cand) : kBuGroup;
-        uint32_t p = pick(pb, k) + skip, p1 = pick(pe, k);
+        uint32_t p = pick(pb, k) + 1, p1 = pick(pe, k);
         uint32_t budget = p + kBuBudget;
         unsigned step = 2;
         while (k < kBuGroup) {  // p < p1 here
@@ -684,7 +683,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
                 if (hit) found |= 1u << k;
                 cand &= ~(1u << k);
                 k = cand ? __builtin_ctz(cand) : kBuGroup;
-                p = pick(pb, k) + skip;
+                p = pick(pb, k) + 1;
                 p1 = pick(pe, k);
                 budget = p + kBuBudget;
                 step = 2;

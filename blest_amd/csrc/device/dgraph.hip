@@ -1,5 +1,6 @@
 // HBM-resident device graph and per-query state: the C ABI handles
 // blest_device_graph / blest_query, and the host-side level-loop driver.
+#include <algorithm>
 #include <cstring>
 
 #include "dgraph.hpp"
@@ -317,41 +318,42 @@ static blest_status create_device_graph(const blest_bvss* bh, const blest_csc* g
                     }
                     g.csrPtr = g.alloc<uint32_t>(size_t(c.n) + 1);
                     BLEST_HIP(hipMemcpy(g.csrPtr, csr->colPtrs.data(), 4 * (size_t(c.n) + 1), hipMemcpyHostToDevice));
-                    g.csrIdx = g.alloc<uint32_t>(csr->nnz());
-                    if (csr->nnz())
-                        BLEST_HIP(hipMemcpy(g.csrIdx, csr->rows.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
-                    g.hasCsr = true;
-                    // vertices without in-neighbours: a bottom-up sweep skips them
                     const std::vector<uint32_t>& cp = csr->colPtrs;
                     const std::vector<uint32_t>& ci = csr->rows;
+                    g.csrIdx = g.alloc<uint32_t>(csr->nnz());
+                    if (opts.bu_probe >= 0) {
+                        // Each vertex's in-neighbours by degree, highest first (ties by ID):
+                        // high-degree vertices are reached first, so a bottom-up scan meets a
+                        // frontier vertex sooner; the first entries also go to `probe`, read
+                        // coalesced by a sweep's first probe instead of one random index line
+                        // per vertex.  Levels do not depend on the scan order, and the
+                        // multi-source sweeps read whole lists.
+                        std::vector<uint32_t> sorted(ci.begin(), ci.end()), probe(c.n, 0u);
+                        parallel_blocks(c.n, hw_threads(), [&](uint64_t v0, uint64_t v1, unsigned) {
+                            for (uint64_t v = v0; v < v1; ++v) {
+                                uint32_t* b0 = sorted.data() + cp[v];
+                                uint32_t* b1 = sorted.data() + cp[v + 1];
+                                std::sort(b0, b1, [&](uint32_t x, uint32_t y) {
+                                    const uint32_t dx = cp[x + 1] - cp[x], dy = cp[y + 1] - cp[y];
+                                    return dx != dy ? dx > dy : x < y;
+                                });
+                                probe[v] = b1 > b0 ? *b0 : 0u;
+                            }
+                        });
+                        if (csr->nnz())
+                            BLEST_HIP(hipMemcpy(g.csrIdx, sorted.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
+                        g.probe = g.alloc<uint32_t>(c.n);
+                        BLEST_HIP(hipMemcpy(g.probe, probe.data(), 4 * size_t(c.n), hipMemcpyHostToDevice));
+                    } else if (csr->nnz()) {
+                        BLEST_HIP(hipMemcpy(g.csrIdx, ci.data(), 4 * csr->nnz(), hipMemcpyHostToDevice));
+                    }
+                    g.hasCsr = true;
+                    // vertices without in-neighbours: a bottom-up sweep skips them
                     std::vector<unsigned long long> noIn(g.W, 0ull);
                     for (uint32_t v = 0; v < c.n; ++v)
                         if (cp[v + 1] == cp[v]) noIn[v >> 6] |= 1ull << (v & 63);
                     g.noIn = g.alloc<unsigned long long>(g.W);
                     BLEST_HIP(hipMemcpy(g.noIn, noIn.data(), 8 * size_t(g.W), hipMemcpyHostToDevice));
-                    if (opts.bu_probe >= 0) {
-                        // the in-neighbour each vertex probes first: its highest-degree one
-                        // (the first such in list order).  High-degree vertices are reached
-                        // first, so at a bottom-up level it is the likeliest to be in the
-                        // frontier, and the sweep reads these probes coalesced instead of
-                        // one random index line per vertex.
-                        std::vector<uint32_t> probe(c.n, 0u);
-                        parallel_blocks(c.n, hw_threads(), [&](uint64_t v0, uint64_t v1, unsigned) {
-                            for (uint64_t v = v0; v < v1; ++v) {
-                                uint32_t best = 0, bestDeg = 0;
-                                for (uint32_t p = cp[v]; p < cp[v + 1]; ++p) {
-                                    const uint32_t u = ci[p], d = cp[u + 1] - cp[u];
-                                    if (p == cp[v] || d > bestDeg) {
-                                        best = u;
-                                        bestDeg = d;
-                                    }
-                                }
-                                probe[v] = best;
-                            }
-                        });
-                        g.probe = g.alloc<uint32_t>(c.n);
-                        BLEST_HIP(hipMemcpy(g.probe, probe.data(), 4 * size_t(c.n), hipMemcpyHostToDevice));
-                    }
                 }
             }
             if (inv_perm) {

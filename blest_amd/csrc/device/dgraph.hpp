@@ -44,8 +44,8 @@ struct GraphView {
     const uint32_t* csrPtr;       // n + 1, bottom-up in-neighbour lists (nullable)
     const uint32_t* csrIdx;       // nnz
     const unsigned long long* noIn;  // W words: vertices without in-neighbours (nullable)
-    const uint32_t* probe;        // n: each vertex's in-neighbour a bottom-up sweep tests first
-                                  // (its highest-degree one; nullable)
+    const uint32_t* probe;        // n: each vertex's first in-neighbour (its highest-degree one:
+                                  // the lists are sorted by degree); nullable
     unsigned long long nnz;       // directed edges
 };
 

@@ -1,11 +1,11 @@
 """CPU emulation of the single-source bottom-up sweep's per-lane logic
 (blest_amd/csrc/device/bfs.hip, bottom-up branch of bfs_expand_kernel): groups of G
-bitmap words per wave, every open vertex of a group probed at once at its probe
-in-neighbour (the highest-degree one, from the per-vertex probe array the device graph
-keeps, or its first list entry without one), then each lane's remaining vertices scanned
-one after another (2, then 4 entries per step; from the list start after a probe, from
-the second entry after a first-entry probe), a scan still open after the budget handed
-to a whole-wave pass through a bounded queue (a full queue keeps the scan on its lane).  The emulation runs the same
+bitmap words per wave, every open vertex of a group probed at its first in-neighbour at
+once (read from the per-vertex probe array when the lists are sorted by degree, as the
+device graph keeps them by default; the same entry), then each lane's remaining vertices
+scanned one after another (2, then 4 entries per step), a scan still open after the
+budget handed to a whole-wave pass through a bounded queue (a full queue keeps the scan
+on its lane).  The emulation runs the same
 control flow lane by lane and checks that the vertices it finds are exactly the
 unvisited vertices with an in-neighbour in the frontier, and that each is found once,
 for both staging depths, tiny budgets and queue capacities that overflow.
@@ -16,16 +16,20 @@ import numpy as np
 import pytest
 
 
-def probe_array(n, cp, idx):
-    """Each vertex's highest-degree in-neighbour, the first such in list order (the rule of
-    create_device_graph in blest_amd/csrc/device/dgraph.hip); 0 for an empty list."""
+def degree_sorted(n, cp, idx):
+    """Each vertex's in-neighbour list by degree, highest first, ties by ID (the order
+    create_device_graph in blest_amd/csrc/device/dgraph.hip uploads), and the probe array
+    (each list's first entry, 0 for an empty list)."""
     deg = np.diff(cp)
-    out = np.zeros(n, dtype=np.int64)
+    out = idx.copy()
+    probe = np.zeros(n, dtype=np.int64)
     for v in range(n):
         nb = idx[cp[v]:cp[v + 1]]
+        nb = nb[np.lexsort((nb, -deg[nb]))]
+        out[cp[v]:cp[v + 1]] = nb
         if len(nb):
-            out[v] = nb[int(np.argmax(deg[nb]))]  # argmax: the first maximum
-    return out
+            probe[v] = nb[0]
+    return out, probe
 
 
 def emulate_sweep(n, cp, idx, visited, frontier, G=8, budget=32, batch=4, heavy_cap=1024, words_per_wave=None,
@@ -46,8 +50,7 @@ def emulate_sweep(n, cp, idx, visited, frontier, G=8, budget=32, batch=4, heavy_
                 if pe[k] > pb[k]:
                     cand |= 1 << k
             fk = 0
-            skip = 0 if probe is not None else 1
-            # first probe of every candidate
+            # first probe of every candidate (the probe array holds the first entries)
             for k in range(len(group)):
                 if (cand >> k) & 1:
                     u = group[k] * 64 + lane
@@ -58,7 +61,7 @@ def emulate_sweep(n, cp, idx, visited, frontier, G=8, budget=32, batch=4, heavy_
                         cand &= ~(1 << k)
             k = (cand & -cand).bit_length() - 1 if cand else len(group)
             if k < len(group):
-                p, p1 = pb[k] + skip, pe[k]
+                p, p1 = pb[k] + 1, pe[k]
                 bud = p + budget
             step = 2
             while k < len(group):
@@ -79,7 +82,7 @@ def emulate_sweep(n, cp, idx, visited, frontier, G=8, budget=32, batch=4, heavy_
                     cand &= ~(1 << k)
                     k = (cand & -cand).bit_length() - 1 if cand else len(group)
                     if k < len(group):
-                        p, p1 = pb[k] + skip, pe[k]
+                        p, p1 = pb[k] + 1, pe[k]
                         bud = p + budget
                     step = 2
             for k, w in enumerate(group):
@@ -120,26 +123,32 @@ def random_case(seed, n=3000, m=20000, p_vis=0.4, p_front=0.05):
 
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("G,budget,heavy_cap", [(8, 32, 1024), (4, 32, 1024), (8, 2, 3), (4, 1, 0)])
-@pytest.mark.parametrize("with_probe", [True, False])
-def test_bottom_up_scan_finds_exactly_the_frontier_neighbours(seed, G, budget, heavy_cap, with_probe):
+@pytest.mark.parametrize("sorted_lists", [True, False])
+def test_bottom_up_scan_finds_exactly_the_frontier_neighbours(seed, G, budget, heavy_cap, sorted_lists):
     n, cp, idx, visited, frontier = random_case(seed)
-    probe = probe_array(n, cp, idx) if with_probe else None
+    probe = None
+    if sorted_lists:
+        idx, probe = degree_sorted(n, cp, idx)
     got = emulate_sweep(n, cp, idx, visited, frontier, G=G, budget=budget, heavy_cap=heavy_cap, probe=probe)
     want = {u for u in range(n) if not visited[u] and any(frontier[idx[q]] for q in range(cp[u], cp[u + 1]))}
     assert got == want
 
 
-def test_probe_is_the_highest_degree_in_neighbour():
-    """Hand-made lists: ties go to the first in list order, and the probe is an
-    in-neighbour of its vertex."""
-    # vertex 0: in-neighbours 1, 2 (degrees 1, 3); vertex 1: 0; vertex 2: 0, 3, 4; 3: 2; 4: 2
+def test_degree_sorted_lists():
+    """Hand-made lists: degree descending, ties by ID; every list keeps its entries and the
+    probe is the highest-degree in-neighbour."""
+    # vertex 0: in-neighbours 1, 2 (degrees 1, 3); vertex 1: 0; vertex 2: 4, 3, 0 (1, 1, 2)
     cp = np.array([0, 2, 3, 6, 7, 8])
-    idx = np.array([1, 2, 0, 0, 3, 4, 2, 2])
-    assert probe_array(5, cp, idx).tolist() == [2, 0, 0, 2, 2]
+    idx = np.array([1, 2, 0, 4, 3, 0, 2, 2])
+    srt, pr = degree_sorted(5, cp, idx)
+    assert srt.tolist() == [2, 1, 0, 0, 3, 4, 2, 2]
+    assert pr.tolist() == [2, 0, 0, 2, 2]
     n, cp, idx, _, _ = random_case(7)
-    pr = probe_array(n, cp, idx)
+    srt, pr = degree_sorted(n, cp, idx)
     deg = np.diff(cp)
     for v in range(n):
-        nb = idx[cp[v]:cp[v + 1]]
-        if len(nb):
-            assert pr[v] in nb and deg[pr[v]] == deg[nb].max()
+        a, b = idx[cp[v]:cp[v + 1]], srt[cp[v]:cp[v + 1]]
+        assert sorted(a.tolist()) == sorted(b.tolist())
+        assert (np.diff(deg[b]) <= 0).all()
+        if len(b):
+            assert pr[v] == b[0] and deg[pr[v]] == deg[a].max()
