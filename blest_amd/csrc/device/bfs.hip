@@ -757,17 +757,37 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     // one batch of independent loads: the flags, the segment's list bases and both
     // candidate words of the first chunk (next for top-down, fout for bottom-up)
     const unsigned done = ld_relaxed(&c->done);
-    const unsigned dir = c->state[L & 1].dir, shift = c->state[L & 1].shift;
+    const unsigned dir = c->state[L & 1].dir, shift = c->state[L & 1].shift, prevNf = c->state[L & 1].prevNf;
     const uint32_t listBase = a.sv.segBase[blockIdx.x], chunkBase = a.sv.chunkBase[blockIdx.x];
     const uint32_t w1 = min(ws + t / 8, g.W - 1);
     unsigned long long* nb64 = reinterpret_cast<unsigned long long*>(a.nextB);
     const unsigned long long firstNB = nb64[size_t(w1) * 8 + b], firstFout = fout[w1];
+    // a bottom-up level's discovery counts are complete (its expand kernel wrote them)
+    unsigned long long nfPart = 0;
+    if (dir != 0)
+        for (uint32_t s = t; s < segs; s += nb) nfPart += a.sc.news[size_t(sOut) * segs + s];
     if (done) return;
     const bool bottomUp = dir != 0;
     const unsigned Le = L + shift;  // the level's distance
     uint32_t* out = ((L & 1) ? a.list0 : a.list1) + listBase;
     uint2* cout = ((L & 1) ? a.chunk0 : a.chunk1) + chunkBase;
     const bool canBU = g.csrPtr != nullptr;
+    if (bottomUp) {
+        // The lists serve a top-down next level only.  Its direction is level_begin's rule
+        // on the same inputs (this level's state and discovery count): when it stays
+        // bottom-up, or nothing was discovered, no list is written.
+        const unsigned long long nf = block_sum_u64(nfPart, tmp64);
+        const bool nextBottomUp = (a.direction == 0 && canBU)
+                                      ? !(double(nf) * a.beta < double(g.n) && nf < prevNf)
+                                      : (a.direction == 2 && canBU);
+        if (nf == 0 || nextBottomUp) {
+            if (t == 0) {
+                a.sc.count[size_t(sOut) * segs + blockIdx.x] = 0u;
+                a.sc.chunks[size_t(sOut) * segs + blockIdx.x] = 0u;
+            }
+            return;
+        }
+    }
     unsigned long long news = 0, edges = 0;
     unsigned written = 0, cwritten = 0;
     for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
