@@ -616,7 +616,8 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
     unsigned long long news = 0, edges = 0;
     for (uint32_t w0 = ws + wib; w0 < we; w0 += wpb * kBuGroup) {
         uint32_t pb[kBuGroup], pe[kBuGroup];
-        unsigned cand = 0;  // bit k: the vertex of word k is open and has in-neighbours
+        unsigned cand = 0;   // bit k: the vertex of word k is open and has in-neighbours
+        unsigned openm = 0;  // bit k: the vertex of word k is open
 #pragma unroll
         for (int k = 0; k < kBuGroup; ++k) {
             const uint32_t w = w0 + k * wpb;
@@ -624,6 +625,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
             const unsigned long long vis = w < we ? (a.visited[w] | (g.noIn ? g.noIn[w] : 0ull)) : ~0ull;
             const uint32_t u = w * 64 + lane;
             const bool open = w < we && u < g.n && !((vis >> lane) & 1ull);
+            if (open) openm |= 1u << k;
             pb[k] = open ? g.csrPtr[u] : 0u;
             pe[k] = open ? g.csrPtr[u + 1] : 0u;
             if (pe[k] > pb[k]) cand |= 1u << k;
@@ -637,12 +639,16 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
         // from its second entry (pb stays the list start: its degree is counted when it
         // is found).
         {
+            // the probe array comes with the no-in-neighbour bitmap, so there an open
+            // vertex has an in-neighbour: its probe goes out with its list bounds, one
+            // round trip before they return
+            const unsigned first = g.probe ? openm : cand;
             uint32_t nb0[kBuGroup], fw0[kBuGroup];
 #pragma unroll
             for (int j = 0; j < kBuGroup; ++j)
-                nb0[j] = ((cand >> j) & 1u) ? (g.probe ? g.probe[(w0 + j * wpb) * 64 + lane] : g.csrIdx[pb[j]]) : 0u;
+                nb0[j] = ((first >> j) & 1u) ? (g.probe ? g.probe[(w0 + j * wpb) * 64 + lane] : g.csrIdx[pb[j]]) : 0u;
 #pragma unroll
-            for (int j = 0; j < kBuGroup; ++j) fw0[j] = ((cand >> j) & 1u) ? fin32[nb0[j] >> 5] : 0u;
+            for (int j = 0; j < kBuGroup; ++j) fw0[j] = ((first >> j) & 1u) ? fin32[nb0[j] >> 5] : 0u;
 #pragma unroll
             for (int j = 0; j < kBuGroup; ++j) {
                 if ((cand >> j) & 1u) {

@@ -27,16 +27,16 @@ struct DirectionParams {
 //
 // Everything is read as one batch of independent loads (the done flag, the state of
 // level L-1, and each counter row once, thread t taking a contiguous run of segments),
-// then reduced and scanned in LDS: a kernel's preamble is one memory round trip, not a
-// chain of them (measured: 8.5 us of a small level's 9-13 us expand kernel went to the
+// then reduced and scanned in one LDS exchange: a kernel's preamble is one memory round
+// trip, not a chain of them (measured: 8.5 us of a small level's 9-13 us expand kernel went to the
 // direction sums and the two prefix loads done one after another).
 __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint32_t segs, unsigned L,
                                             const DirectionParams& p, bool& bottomUp, unsigned* pre = nullptr,
                                             unsigned* total = nullptr, unsigned* cpre = nullptr,
                                             unsigned* ctotal = nullptr, unsigned* effLevel = nullptr,
                                             LevelState* outState = nullptr, LevelState* inState = nullptr) {
-    __shared__ unsigned long long s_tmp[16];
-    __shared__ unsigned s_tmp32[16];
+    __shared__ unsigned long long s_tmp[16], s_etmp[16];
+    __shared__ unsigned s_tmp32[16], s_ctmp32[16];
     const unsigned t = threadIdx.x, nb = blockDim.x;
     const unsigned per = (segs + nb - 1) / nb;
     const size_t base = size_t(L % 3) * segs;
@@ -71,15 +71,42 @@ __device__ __forceinline__ bool level_begin(Ctrl* c, const SegCounters& sc, uint
     }
     // levels enqueued past the end of the traversal stop here, before any reduction
     if (__syncthreads_or(doneIn != 0)) return false;
-    nf = block_sum_u64(nf, s_tmp);
-    mf = block_sum_u64(mf, s_tmp);
-    if (pre) {
-        const unsigned ex = block_exclusive_scan(mine, s_tmp32, *total);
-        for (unsigned j = 0; j < per && t * per + j < segs; ++j) pre[t * per + j] += ex;
-    }
-    if (cpre) {
-        const unsigned ex = block_exclusive_scan(cmine, s_tmp32, *ctotal);
-        for (unsigned j = 0; j < per && t * per + j < segs; ++j) cpre[t * per + j] += ex;
+    // the two sums and the two scans share one exchange through LDS (one barrier, not
+    // eight): each wave posts its totals, every thread then adds up the waves before it
+    {
+        const unsigned lane = lane_id(), wave = t / kWave, nw = nb / kWave;
+        const unsigned long long wnf = wave_sum_u64(nf), wmf = wave_sum_u64(mf);
+        const unsigned incl = wave_inclusive_scan(mine), cincl = wave_inclusive_scan(cmine);
+        if (lane == kWave - 1) {
+            s_tmp32[wave] = incl;
+            s_ctmp32[wave] = cincl;
+        }
+        if (lane == 0) {
+            s_tmp[wave] = wnf;
+            s_etmp[wave] = wmf;
+        }
+        __syncthreads();
+        nf = mf = 0;
+        unsigned before = 0, cbefore = 0, tot = 0, ctot = 0;
+        for (unsigned i = 0; i < nw; ++i) {
+            nf += s_tmp[i];
+            mf += s_etmp[i];
+            const unsigned x = s_tmp32[i], y = s_ctmp32[i];
+            before += i < wave ? x : 0u;
+            cbefore += i < wave ? y : 0u;
+            tot += x;
+            ctot += y;
+        }
+        if (pre) {
+            *total = tot;
+            const unsigned ex = before + incl - mine;
+            for (unsigned j = 0; j < per && t * per + j < segs; ++j) pre[t * per + j] += ex;
+        }
+        if (cpre) {
+            *ctotal = ctot;
+            const unsigned ex = cbefore + cincl - cmine;
+            for (unsigned j = 0; j < per && t * per + j < segs; ++j) cpre[t * per + j] += ex;
+        }
     }
     // the prefixes are complete and every thread has read state / counters before
     // block 0 writes
