@@ -744,16 +744,23 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
 // words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
-// vsets (one block scan per chunk, entries written by all threads: binary search for a
-// thread's first owner, then a galloping search).  All of a thread's loads in a chunk are independent, so a level costs a few
-// memory round trips whatever the frontier size.
+// vsets (one block scan per chunk for both lists, entries written by all threads: binary
+// search for a thread's first owner, then a galloping search).  A chunk's loads are
+// independent of each other as far as the data allows: the discovery bytes with the
+// visited word, then the active rsets' vset ranges, then the new vertices' degrees, all
+// of a thread's eight vertices at once (issued in one pass with every load in flight
+// before the first is used; one round trip per vertex made the dense levels of deep
+// graphs cost ~10 us per chunk).
+constexpr int kDegBatch = 8;  // degree loads in flight per thread: all of its vertices in a chunk
 __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
-    __shared__ unsigned tmp[16];
+    __shared__ unsigned tmp[32];
     __shared__ unsigned long long tmp64[16];
     __shared__ unsigned sEx[kProducerBlock + 1];
     __shared__ uint32_t sStart[kProducerBlock];
-    __shared__ unsigned long long sWord[kProducerBlock / 8];
+    // the chunk's new-vertex words, by chunk parity: a chunk's levels are written after its
+    // lists, while faster waves may already store the next chunk's words
+    __shared__ unsigned long long sWord[2][kProducerBlock / 8];
     Ctrl* c = a.ctrl;
     unsigned long long* fout = (L & 1) ? a.front0 : a.front1;
     const uint32_t segs = a.sv.segs;
@@ -795,9 +802,15 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         }
     }
     unsigned long long news = 0, edges = 0;
-    unsigned written = 0, cwritten = 0;
-    for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
-        const uint32_t w = c0 + t / 8, r = w * 8 + b;
+    unsigned written = 0, cwritten = 0, pass = 0;
+    // the thread's candidate (next bytes, or the frontier word) of the chunk about to run:
+    // each chunk loads the next chunk's before its own work (one load in flight throughout)
+    unsigned long long pre = ws + t / 8 < we ? (bottomUp ? firstFout : firstNB) : 0ull;
+    for (uint32_t c0 = ws; c0 < we; c0 += nb / 8, ++pass) {
+        unsigned long long* sw = sWord[pass & 1];
+        const uint32_t w = c0 + t / 8, r = w * 8 + b, wn = w + nb / 8;
+        const unsigned long long cur = pre;
+        pre = wn < we ? (bottomUp ? fout[wn] : nb64[size_t(wn) * 8 + b]) : 0ull;
         uint32_t start = 0, cnt = 0, chunks = 0;
         bool any = false;
         unsigned long long word = 0;
@@ -805,7 +818,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             // the word's discoveries: thread b holds bytes [64w + 8b, 64w + 8b + 8) of
             // nextB, turns them into bits 8b..8b+7, and the word's 8 threads (aligned lanes
             // of one wave) OR them together; each clears the bytes it read
-            const unsigned long long raw = w < we ? (c0 == ws ? firstNB : nb64[size_t(w) * 8 + b]) : 0ull;
+            const unsigned long long raw = cur;
             unsigned m8 = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) m8 |= ((raw >> (8 * i)) & 0xFFull) ? (1u << i) : 0u;
@@ -817,11 +830,11 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         }
         if (w < we) {
             if (bottomUp) {
-                word = c0 == ws ? firstFout : fout[w];
+                word = cur;
             } else if (b == 0) {
-                if (word) a.visited[w] |= word;
+                if (word) atomicOr(&a.visited[w], word);  // result unused: no round trip
                 fout[w] = word;
-                sWord[t / 8] = word;
+                sw[t / 8] = word;
             }
             const uint32_t byte = uint32_t(word >> (8 * b)) & 0xFFu;
             any = byte != 0;
@@ -831,27 +844,15 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
                 chunks = rset_chunks(cnt);  // a hub's rset: chunk entries, not vsets
             }
         } else if (!bottomUp && b == 0) {
-            sWord[t / 8] = 0ull;
+            sw[t / 8] = 0ull;
         }
         // a chunk without any frontier vertex (most of them on a narrow level) needs no
         // scan, levels or list: one block-wide vote instead
         if (!__syncthreads_or(any)) continue;
-        unsigned total;
-        const unsigned ex = block_exclusive_scan(chunks ? 0u : cnt, tmp, total);  // its barriers publish sWord
-        if (!bottomUp) {
-            // the new vertices' levels and degrees, thread per vertex (coalesced): the
-            // chunk's 64 words hold 8 x blockDim vertices
-#pragma unroll 4  // (8 in flight spilled past the 64 VGPRs of a resident grid)
-            for (int i = 0; i < 8; ++i) {
-                const unsigned li = t + i * nb;  // vertex index within the chunk
-                if ((sWord[li >> 6] >> (li & 63)) & 1ull) {
-                    const uint32_t v = c0 * 64 + li;
-                    a.levels[v] = int32_t(Le);
-                    if (canBU) edges += g.csrPtr[v + 1] - g.csrPtr[v];
-                    news += 1;
-                }
-            }
-        }
+        unsigned total, cex, ctotal;
+        // (its barriers publish sw, and order this chunk's sEx / sStart stores after the
+        // previous chunk's readers)
+        const unsigned ex = block_exclusive_scan2(chunks ? 0u : cnt, chunks, tmp, total, cex, ctotal);
         sEx[t] = ex;  // rsets listed as chunks own no entries here
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
@@ -862,11 +863,35 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
         written += total;
-        unsigned ctotal;
-        const unsigned cex = block_exclusive_scan(chunks, tmp, ctotal);  // also orders sEx / sStart reuse
         for (uint32_t j = 0; j < chunks; ++j)
             cout[cwritten + cex + j] = make_uint2(start + j * kChunkVsets, min(kChunkVsets, cnt - j * kChunkVsets));
         cwritten += ctotal;
+        if (!bottomUp) {
+            // the new vertices' levels and degrees, lane per vertex (coalesced): the
+            // chunk's 64 words hold 8 x blockDim vertices, word wv + i x (waves per block)
+            // for the wave's i-th step, taken kDegBatch steps at a time
+            const unsigned wv = __builtin_amdgcn_readfirstlane(t / kWave), lane = t % kWave;
+#pragma unroll
+            for (int i0 = 0; i0 < 8; i0 += kDegBatch) {
+                uint32_t lo[kDegBatch], hi[kDegBatch];
+#pragma unroll
+                for (int j = 0; j < kDegBatch; ++j) {
+                    const unsigned wi = wv + (i0 + j) * (nb / kWave);  // word within the chunk
+                    const uint32_t v = (c0 + wi) * 64 + lane;
+                    lo[j] = hi[j] = 0u;
+                    if ((sw[wi] >> lane) & 1ull) {
+                        a.levels[v] = int32_t(Le);
+                        news += 1;
+                        if (canBU) {
+                            lo[j] = g.csrPtr[v];
+                            hi[j] = g.csrPtr[v + 1];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kDegBatch; ++j) edges += hi[j] - lo[j];
+            }
+        }
     }
     if (!bottomUp) {
         news = block_sum_u64(news, tmp64);

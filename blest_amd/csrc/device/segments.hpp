@@ -72,6 +72,32 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned x, unsigned* t
     return before + incl - x;
 }
 
+// Two exclusive scans over the block through one LDS exchange (the barriers of one
+// scan): returns x's exclusive prefix, y's in ey, and the totals.  `tmp` holds >= 32.
+__device__ __forceinline__ unsigned block_exclusive_scan2(unsigned x, unsigned y, unsigned* tmp, unsigned& total,
+                                                          unsigned& ey, unsigned& ytotal) {
+    const unsigned lane = lane_id(), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    const unsigned ix = wave_inclusive_scan(x), iy = wave_inclusive_scan(y);
+    __syncthreads();
+    if (lane == kWave - 1) {
+        tmp[wave] = ix;
+        tmp[16 + wave] = iy;
+    }
+    __syncthreads();
+    unsigned bx = 0, by = 0, ax = 0, ay = 0;
+    for (unsigned i = 0; i < nw; ++i) {
+        const unsigned tx = tmp[i], ty = tmp[16 + i];
+        bx += (i < wave) ? tx : 0u;
+        by += (i < wave) ? ty : 0u;
+        ax += tx;
+        ay += ty;
+    }
+    total = ax;
+    ytotal = ay;
+    ey = by + iy - y;
+    return bx + ix - x;
+}
+
 __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x, unsigned long long* tmp) {
     const unsigned lane = lane_id(), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
     x = wave_sum_u64(x);
