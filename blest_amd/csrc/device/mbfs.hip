@@ -116,7 +116,7 @@ __device__ __forceinline__ unsigned long long* mfront(const MbfsArgs& a, uint32_
 // written by the whole workgroup (block_write_vsets); each chunk's 64 word activities
 // are first staged in LDS (wave w computes words 8w .. 8w+7).  Returns the count.
 struct ListLds {
-    unsigned tmp[16];
+    unsigned tmp[32];
     unsigned ex[kProducerBlock + 1];
     uint32_t start[kProducerBlock];
     unsigned long long act[kProducerBlock / 8];
@@ -720,19 +720,20 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
             }
         }
     }
-    __syncthreads();  // the activity words above are read by other waves below
+    // the counts first (their barriers also publish the activity words to the other waves),
+    // so that nothing of the word pass stays live through the list writing
+    news = block_sum_u64(news, tmp64);
+    edges = block_sum_u64(edges, tmp64);
+    if (threadIdx.x == 0) {
+        a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(min(news, 0xFFFFFFFFull));
+        a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;
+    }
     const unsigned long long* anyL = a.any + size_t(L & 1) * g.W;
     const uint32_t ws2 = blockIdx.x * a.sv.wps, we2 = min(g.W, ws2 + a.sv.wps);
     const unsigned running = block_write_vsets(
         g.rsetUnit, ws2, we2, [](uint32_t) {}, [&](uint32_t w) { return anyL[w]; }, lout + a.sv.segBase[blockIdx.x],
         l.ex, l.start, l.tmp);
-    news = block_sum_u64(news, tmp64);
-    edges = block_sum_u64(edges, tmp64);
-    if (threadIdx.x == 0) {
-        a.sc.count[size_t(sOut) * segs + blockIdx.x] = running;
-        a.sc.news[size_t(sOut) * segs + blockIdx.x] = unsigned(min(news, 0xFFFFFFFFull));
-        a.sc.edges[size_t(sOut) * segs + blockIdx.x] = edges;
-    }
+    if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = running;
 }
 
 // Every (root, vertex) level at once, thread per vertex.  The level of root j is the

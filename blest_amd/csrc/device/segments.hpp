@@ -240,14 +240,21 @@ __device__ __forceinline__ unsigned owner_from(const unsigned* ex, unsigned nb, 
 // of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
 // are written by all threads, entry e by thread e % blockDim (coalesced); a thread finds
 // the owner of its first entry by binary search over the chunk's prefix in LDS and of
-// each later one with owner_from.  A segment holding hub rsets
-// (hundreds of thousands of vsets) is so spread over the whole workgroup.  prep(c0) runs
-// on every thread before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds
-// blockDim + 1 entries, sStart blockDim.  Returns the segment's entry count.
+// each later one with owner_from.  An rset of at least kRunVsets vsets (a hub's, up to
+// thousands; Gorder puts hubs side by side, so one segment can hold 10^5 vsets of them)
+// is written after the others as a run, start, start + 1, ..., by the whole workgroup
+// with no owner search (measured: a multi-source update over a segment of 1.7e5 hub
+// vsets took 130-150 us with the per-entry search).  prep(c0) runs on every thread
+// before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds blockDim + 1
+// entries, sStart blockDim, tmp 32.  Returns the segment's entry count.
+constexpr uint32_t kRunVsets = 256;
+
 template <class Prep, class BitsOf>
 __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, uint32_t ws, uint32_t we, Prep prep,
                                                       BitsOf bitsOf, uint32_t* out, unsigned* sEx, uint32_t* sStart,
                                                       unsigned* tmp) {
+    __shared__ uint32_t sRunStart[kProducerBlock], sRunCnt[kProducerBlock], sRunOff[kProducerBlock];
+    __shared__ unsigned sRuns;
     const unsigned t = threadIdx.x, nb = blockDim.x;
     unsigned written = 0;
     for (uint32_t c0 = ws; c0 < we; c0 += nb / 8) {
@@ -261,19 +268,33 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
                 cnt = rsetUnit[w * 8 + b + 1] - start;
             }
         }
-        unsigned total;
-        const unsigned ex = block_exclusive_scan(cnt, tmp, total);
-        sEx[t] = ex;
+        if (t == 0) sRuns = 0;  // ordered before the claims below by the scan's barriers
+        const bool run = cnt >= kRunVsets;
+        unsigned total, rex, rtotal;
+        const unsigned ex = block_exclusive_scan2(run ? 0u : cnt, run ? cnt : 0u, tmp, total, rex, rtotal);
+        sEx[t] = ex;  // runs own no entries of the searched part
         sStart[t] = start;
         if (t == 0) sEx[nb] = total;
+        if (run) {
+            const unsigned k = atomicAdd(&sRuns, 1u);  // queue order is free: offsets come from the scan
+            sRunStart[k] = start;
+            sRunCnt[k] = cnt;
+            sRunOff[k] = total + rex;
+        }
         __syncthreads();
         unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
         for (unsigned e = t; e < total; e += nb) {
             o = owner_from(sEx, nb, o, e);
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
-        written += total;
-        __syncthreads();  // the next chunk reuses sEx / sStart
+        const unsigned nr = sRuns;
+        for (unsigned k = 0; k < nr; ++k) {
+            const uint32_t s0 = sRunStart[k], c = sRunCnt[k];
+            uint32_t* o2 = out + written + sRunOff[k];
+            for (uint32_t j = t; j < c; j += nb) o2[j] = s0 + j;
+        }
+        written += total + rtotal;
+        __syncthreads();  // the next chunk reuses sEx / sStart / the run queue
     }
     return written;
 }

@@ -10,9 +10,25 @@ if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
 
+_capture = None
+
+
 def pytest_configure(config):
+    global _capture
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
     config.addinivalue_line("markers", "slow: larger CPU cases")
+    _capture = config.pluginmanager.getplugin("capturemanager")
+
+
+def progress(*a):
+    """A progress line that pytest's output capture does not hold back: a long case (the
+    R-MAT 26 one prepares its graph for minutes) shows as it goes that it is running, so
+    a runner that takes minutes of silence for a hang sees it alive."""
+    if _capture is not None:
+        with _capture.global_and_fixture_disabled():
+            print(*a, flush=True)
+    else:
+        print(*a, flush=True)
 
 
 def _ensure_built():

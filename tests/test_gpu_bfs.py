@@ -5,13 +5,13 @@ import sys
 
 import numpy as np
 import pytest
+from conftest import progress
 
 pytestmark = pytest.mark.gpu
 
 
 def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
+    progress(*a)
 
 
 def pipeline(blest, make, reorder=None, window=1024, full_padding=None, **dev):

@@ -7,13 +7,13 @@ import time
 
 import numpy as np
 import pytest
+from conftest import progress
 
 pytestmark = pytest.mark.gpu
 
 
 def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
+    progress(*a)
 
 
 def assert_same(host, dev, name):

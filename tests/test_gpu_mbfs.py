@@ -4,13 +4,13 @@ import sys
 
 import numpy as np
 import pytest
+from conftest import progress
 
 pytestmark = pytest.mark.gpu
 
 
 def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
+    progress(*a)
 
 
 @pytest.mark.parametrize("algo", [0, 1])

@@ -11,6 +11,7 @@ import time
 
 import numpy as np
 import pytest
+from conftest import progress
 
 pytestmark = pytest.mark.gpu
 
@@ -18,8 +19,7 @@ _prepared = {}
 
 
 def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
+    progress(*a)
 
 
 def workload(name):

@@ -9,6 +9,7 @@ import sys
 
 import numpy as np
 import pytest
+from conftest import progress
 
 pytestmark = pytest.mark.gpu
 
@@ -16,8 +17,7 @@ HIP_D2H = 2
 
 
 def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
+    progress(*a)
 
 
 class DeviceBuffer:
