@@ -293,6 +293,9 @@ def mbfs_bench(a, prep, opts, dg=None, algos=(0,)) -> dict:
         plain_ms.append(st.kernel_ms)
         mc.append(st.edges_traversed_component)
     del q1
+    # The plain leg ran at the graph's (tuned) single-source thresholds; the batched leg
+    # switches on (vertex, root) pairs with its own thresholds (--mbfs-alpha/--mbfs-beta).
+    dg.set_thresholds(a.mbfs_alpha, a.mbfs_beta, int(opts.get("direction", 0)))
     qm = B._Query(dg, width)
     batches = [np.ascontiguousarray(np.array(roots[i:i + width], dtype=np.uint32)) for i in range(0, len(roots), width)]
     total_mc = float(sum(mc))
@@ -310,6 +313,8 @@ def mbfs_bench(a, prep, opts, dg=None, algos=(0,)) -> dict:
         gt = total_mc * a.steps / (sum(ms) * 1e-3) / 1e9
         log(f"[mbfs] algo {algo}: plain {plain:.2f} GTEPS ({np.mean(plain_ms):.3f} ms/root), batched {gt:.2f} GTEPS "
             f"({np.mean(ms):.3f} ms per {width}-root batch): {gt / plain:.2f}x")
+    del qm
+    dg.set_thresholds(float(opts.get("alpha", 15.0)), float(opts.get("beta", 18.0)), int(opts.get("direction", 0)))
     batch_ms = per_algo[algos[0]]
     batched = total_mc * a.steps / (sum(batch_ms) * 1e-3) / 1e9
     others = {str(k): {"gteps": round(total_mc * a.steps / (sum(v) * 1e-3) / 1e9, 3),
@@ -322,7 +327,8 @@ def mbfs_bench(a, prep, opts, dg=None, algos=(0,)) -> dict:
             "ms_per_step": round(float(sum(batch_ms)) / a.steps, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64", "data": "synthetic",
             "config": {"workload": prep.workload.name + "_mbfs", "vertices": info["n"], "directed_edges": info["nnz"],
-                       "roots": len(roots), "batch": width, "reorder": prep.workload.reorder.name.lower()},
+                       "roots": len(roots), "batch": width, "reorder": prep.workload.reorder.name.lower(),
+                       "alpha": a.mbfs_alpha, "beta": a.mbfs_beta},
             "plain_gteps": round(plain, 3), "speedup_vs_plain": round(batched / plain, 2),
             "plain_ms_per_root": round(float(np.mean(plain_ms)), 4),
             "batch_ms": round(float(np.mean(batch_ms)), 4), "by_algo": others, "roofline": None,
@@ -446,6 +452,8 @@ def main():
     ap.add_argument("--mbfs-algo", type=lambda x: [int(v) for v in x.split(",")], default=[0],
                     help="multi-source products to time: 0 bit-parallel VALU, 1 i8 MFMA (comma list)")
     ap.add_argument("--mbfs-roots", type=int, default=1024)
+    ap.add_argument("--mbfs-alpha", type=float, default=15.0, help="multi-source switch threshold alpha")
+    ap.add_argument("--mbfs-beta", type=float, default=18.0, help="multi-source switch threshold beta")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-plain-sample", type=int, default=128)
@@ -658,7 +666,7 @@ def main():
         mb = {"gteps": r["value"], "plain_gteps": r["plain_gteps"], "speedup_vs_plain": r["speedup_vs_plain"],
               "roots": r["config"]["roots"], "batch": r["config"]["batch"], "batch_ms": r["batch_ms"],
               "plain_ms_per_root": r["plain_ms_per_root"], "algo": "bit-parallel",
-              "i8_mfma": r["by_algo"]["1"]}
+              "alpha": r["config"]["alpha"], "beta": r["config"]["beta"], "i8_mfma": r["by_algo"]["1"]}
     info = dg.info()
     cfg = {"workload": w.name}
     if w.kind == "rmat":
