@@ -242,12 +242,13 @@ __device__ __forceinline__ unsigned owner_from(const unsigned* ex, unsigned nb, 
 // the owner of its first entry by binary search over the chunk's prefix in LDS and of
 // each later one with owner_from.  An rset of at least kRunVsets vsets (a hub's, up to
 // thousands; Gorder puts hubs side by side, so one segment can hold 10^5 vsets of them)
-// is written after the others as a run, start, start + 1, ..., by the whole workgroup
-// with no owner search (measured: a multi-source update over a segment of 1.7e5 hub
-// vsets took 130-150 us with the per-entry search).  prep(c0) runs on every thread
+// is written after the others as a run, start, start + 1, ..., by one wave with no owner
+// search: a thread's entries lie blockDim apart, so in a segment of rsets of ~160 vsets
+// each the owner moves every step and each step costs a galloping search of several
+// dependent LDS reads.  prep(c0) runs on every thread
 // before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds blockDim + 1
 // entries, sStart blockDim, tmp 32.  Returns the segment's entry count.
-constexpr uint32_t kRunVsets = 256;
+constexpr uint32_t kRunVsets = 32;
 
 template <class Prep, class BitsOf>
 __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, uint32_t ws, uint32_t we, Prep prep,
@@ -287,11 +288,13 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
             o = owner_from(sEx, nb, o, e);
             out[written + e] = sStart[o] + (e - sEx[o]);
         }
-        const unsigned nr = sRuns;
-        for (unsigned k = 0; k < nr; ++k) {
+        // runs dealt out to the waves, each written by one wave's lanes (a run of a few
+        // dozen vsets keeps one wave busy, not the whole workgroup stepping through it)
+        const unsigned nr = sRuns, wave = t / kWave, nwv = nb / kWave, lane = t % kWave;
+        for (unsigned k = wave; k < nr; k += nwv) {
             const uint32_t s0 = sRunStart[k], c = sRunCnt[k];
             uint32_t* o2 = out + written + sRunOff[k];
-            for (uint32_t j = t; j < c; j += nb) o2[j] = s0 + j;
+            for (uint32_t j = lane; j < c; j += kWave) o2[j] = s0 + j;
         }
         written += total + rtotal;
         __syncthreads();  // the next chunk reuses sEx / sStart / the run queue
