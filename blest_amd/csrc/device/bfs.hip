@@ -744,8 +744,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
 // words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
-// vsets (one block scan per chunk for both lists, entries written by all threads: binary
-// search for a thread's first owner, then a galloping search).  A chunk's loads are
+// vsets (one block scan per chunk for both lists; each rset writes its own entries).  A chunk's loads are
 // independent of each other as far as the data allows: the discovery bytes with the
 // visited word, then the active rsets' vset ranges, then the new vertices' degrees, all
 // of a thread's eight vertices at once (issued in one pass with every load in flight
@@ -756,8 +755,6 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
     __shared__ unsigned tmp[32];
     __shared__ unsigned long long tmp64[16];
-    __shared__ unsigned sEx[kProducerBlock + 1];
-    __shared__ uint32_t sStart[kProducerBlock];
     // the chunk's new-vertex words, by chunk parity: a chunk's levels are written after its
     // lists, while faster waves may already store the next chunk's words
     __shared__ unsigned long long sWord[2][kProducerBlock / 8];
@@ -850,18 +847,12 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update
         // scan, levels or list: one block-wide vote instead
         if (!__syncthreads_or(any)) continue;
         unsigned total, cex, ctotal;
-        // (its barriers publish sw, and order this chunk's sEx / sStart stores after the
-        // previous chunk's readers)
-        const unsigned ex = block_exclusive_scan2(chunks ? 0u : cnt, chunks, tmp, total, cex, ctotal);
-        sEx[t] = ex;  // rsets listed as chunks own no entries here
-        sStart[t] = start;
-        if (t == 0) sEx[nb] = total;
-        __syncthreads();
-        unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
-        for (unsigned e = t; e < total; e += nb) {
-            o = owner_from(sEx, nb, o, e);
-            out[written + e] = sStart[o] + (e - sEx[o]);
-        }
+        const unsigned ex = block_exclusive_scan2(chunks ? 0u : cnt, chunks, tmp, total, cex, ctotal);  // publishes sw
+        // an rset listed per vset (at most kChunkVsets) writes its own entries: no owner
+        // search (a thread's entries lay a workgroup apart, so each was a search of
+        // dependent LDS reads) and no barrier
+        if (!chunks)
+            for (uint32_t j = 0; j < cnt; ++j) out[written + ex + j] = start + j;
         written += total;
         for (uint32_t j = 0; j < chunks; ++j)
             cout[cwritten + cex + j] = make_uint2(start + j * kChunkVsets, min(kChunkVsets, cnt - j * kChunkVsets));

@@ -117,8 +117,6 @@ __device__ __forceinline__ unsigned long long* mfront(const MbfsArgs& a, uint32_
 // are first staged in LDS (wave w computes words 8w .. 8w+7).  Returns the count.
 struct ListLds {
     unsigned tmp[32];
-    unsigned ex[kProducerBlock + 1];
-    uint32_t start[kProducerBlock];
     unsigned long long act[kProducerBlock / 8];
 };
 
@@ -148,7 +146,7 @@ __device__ __forceinline__ unsigned segment_list(const GraphView& g, const SegVi
     // chunks start at ws + k * blockDim/8, so a word's place in its chunk is (w - ws) % (blockDim/8)
     return block_write_vsets(
         g.rsetUnit, ws, we, prep, [&](uint32_t w) { return l.act[(w - ws) % (blockDim.x / 8)]; },
-        list + sv.segBase[blockIdx.x], l.ex, l.start, l.tmp);
+        list + sv.segBase[blockIdx.x], l.tmp);
 }
 
 template <int NW>
@@ -732,7 +730,7 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID mbfs_updat
     const uint32_t ws2 = blockIdx.x * a.sv.wps, we2 = min(g.W, ws2 + a.sv.wps);
     const unsigned running = block_write_vsets(
         g.rsetUnit, ws2, we2, [](uint32_t) {}, [&](uint32_t w) { return anyL[w]; }, lout + a.sv.segBase[blockIdx.x],
-        l.ex, l.start, l.tmp);
+        l.tmp);
     if (threadIdx.x == 0) a.sc.count[size_t(sOut) * segs + blockIdx.x] = running;
 }
 

@@ -198,62 +198,21 @@ __device__ __forceinline__ unsigned wave_write_vsets(const uint32_t* rsetUnit, u
     return written;
 }
 
-// Largest r in [0, nb) with ex[r] <= e, given ex ascending, ex[0] == 0 and
-// ex[nb] > e: the rset that owns entry e (rsets with no entries are skipped).
-__device__ __forceinline__ unsigned block_owner(const unsigned* ex, unsigned nb, unsigned e) {
-    unsigned lo = 0, hi = nb;
-    while (hi - lo > 1) {
-        const unsigned mid = (lo + hi) >> 1;
-        if (ex[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Owner of entry e >= the first entry of rset o (same ex as block_owner): gallops 1, 2,
-// 4, ... rsets ahead of o, then bisects.  A thread's next entry lies blockDim entries
-// further on: inside a hub's run of vsets the owner does not move (one LDS read), and
-// across a run of small rsets (about two vsets each on R-MAT graphs, so ~blockDim/2
-// rsets) it costs O(log) reads instead of one read per rset.
-__device__ __forceinline__ unsigned owner_from(const unsigned* ex, unsigned nb, unsigned o, unsigned e) {
-    if (ex[o + 1] > e) return o;
-    unsigned lo = o + 1, hi = nb, step = 1;  // ex[lo] <= e < ex[nb]
-    for (;;) {
-        const unsigned probe = lo + step;
-        if (probe >= nb) break;
-        if (ex[probe] > e) {
-            hi = probe;
-            break;
-        }
-        lo = probe;
-        step <<= 1;
-    }
-    while (hi - lo > 1) {
-        const unsigned mid = (lo + hi) >> 1;
-        if (ex[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Block-wide form for a producer workgroup's whole segment [ws, we): thread = one rset
-// of blockDim/8 consecutive words per chunk; after one block scan the chunk's entries
-// are written by all threads, entry e by thread e % blockDim (coalesced); a thread finds
-// the owner of its first entry by binary search over the chunk's prefix in LDS and of
-// each later one with owner_from.  An rset of at least kRunVsets vsets (a hub's, up to
-// thousands; Gorder puts hubs side by side, so one segment can hold 10^5 vsets of them)
-// is written after the others as a run, start, start + 1, ..., by one wave with no owner
-// search: a thread's entries lie blockDim apart, so in a segment of rsets of ~160 vsets
-// each the owner moves every step and each step costs a galloping search of several
-// dependent LDS reads.  prep(c0) runs on every thread
-// before a chunk's bits are read (e.g. to stage them in LDS).  sEx holds blockDim + 1
-// entries, sStart blockDim, tmp 32.  Returns the segment's entry count.
+// of blockDim/8 consecutive words per chunk; after one block scan each rset of fewer than
+// kRunVsets vsets writes its own entries (a handful on R-MAT graphs), and the larger ones
+// (a hub's: up to thousands; Gorder puts hubs side by side, so one segment can hold 10^5
+// vsets of them) are written after them as runs, start, start + 1, ..., dealt out to the
+// waves, each run written by one wave's lanes.  (An owner search over the chunk's prefix
+// in LDS, every thread writing entries a workgroup apart, cost a galloping search of
+// dependent LDS reads per entry: a 128-root update over a segment of 1.7e5 hub vsets took
+// 130-150 us.)  prep(c0) runs on every thread before a chunk's bits are read (e.g. to
+// stage them in LDS).  tmp holds 32 entries.  Returns the segment's entry count.
 constexpr uint32_t kRunVsets = 32;
 
 template <class Prep, class BitsOf>
 __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, uint32_t ws, uint32_t we, Prep prep,
-                                                      BitsOf bitsOf, uint32_t* out, unsigned* sEx, uint32_t* sStart,
-                                                      unsigned* tmp) {
+                                                      BitsOf bitsOf, uint32_t* out, unsigned* tmp) {
     __shared__ uint32_t sRunStart[kProducerBlock], sRunCnt[kProducerBlock], sRunOff[kProducerBlock];
     __shared__ unsigned sRuns;
     const unsigned t = threadIdx.x, nb = blockDim.x;
@@ -273,23 +232,15 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
         const bool run = cnt >= kRunVsets;
         unsigned total, rex, rtotal;
         const unsigned ex = block_exclusive_scan2(run ? 0u : cnt, run ? cnt : 0u, tmp, total, rex, rtotal);
-        sEx[t] = ex;  // runs own no entries of the searched part
-        sStart[t] = start;
-        if (t == 0) sEx[nb] = total;
         if (run) {
             const unsigned k = atomicAdd(&sRuns, 1u);  // queue order is free: offsets come from the scan
             sRunStart[k] = start;
             sRunCnt[k] = cnt;
             sRunOff[k] = total + rex;
+        } else {
+            for (uint32_t j = 0; j < cnt; ++j) out[written + ex + j] = start + j;
         }
         __syncthreads();
-        unsigned o = t < total ? block_owner(sEx, nb, t) : 0u;
-        for (unsigned e = t; e < total; e += nb) {
-            o = owner_from(sEx, nb, o, e);
-            out[written + e] = sStart[o] + (e - sEx[o]);
-        }
-        // runs dealt out to the waves, each written by one wave's lanes (a run of a few
-        // dozen vsets keeps one wave busy, not the whole workgroup stepping through it)
         const unsigned nr = sRuns, wave = t / kWave, nwv = nb / kWave, lane = t % kWave;
         for (unsigned k = wave; k < nr; k += nwv) {
             const uint32_t s0 = sRunStart[k], c = sRunCnt[k];
@@ -297,7 +248,7 @@ __device__ __forceinline__ unsigned block_write_vsets(const uint32_t* rsetUnit, 
             for (uint32_t j = lane; j < c; j += kWave) o2[j] = s0 + j;
         }
         written += total + rtotal;
-        __syncthreads();  // the next chunk reuses sEx / sStart / the run queue
+        __syncthreads();  // the next chunk reuses the run queue
     }
     return written;
 }
