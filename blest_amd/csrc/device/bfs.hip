@@ -744,12 +744,12 @@ __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_expand
 // words (one rset per thread, the 8 rsets of a word in one wave).  Top-down levels
 // fold `next` into visited / frontier (the word's first thread), write the new
 // vertices' levels and sum their degrees; both directions then emit the active rsets'
-// vsets (one block scan per chunk for both lists; each rset writes its own entries).  A chunk's loads are
-// independent of each other as far as the data allows: the discovery bytes with the
-// visited word, then the active rsets' vset ranges, then the new vertices' degrees, all
-// of a thread's eight vertices at once (issued in one pass with every load in flight
-// before the first is used; one round trip per vertex made the dense levels of deep
-// graphs cost ~10 us per chunk).
+// vsets (one block scan per chunk for both lists; each rset writes its own entries).
+// A chunk's loads are as independent as the data allows: its discovery bytes were
+// loaded during the previous chunk, then come the active rsets' vset ranges, then the
+// degrees of all of a thread's eight new vertices at once (every load in flight before
+// the first is used; one round trip per vertex made the dense levels of deep graphs cost
+// ~10 us per chunk).
 constexpr int kDegBatch = 8;  // degree loads in flight per thread: all of its vertices in a chunk
 __global__ void __launch_bounds__(kProducerBlock) BLEST_RESIDENT_GRID bfs_update_kernel(GraphView g, BfsArgs a, unsigned Lk) {
     const unsigned L = level_of(a.ctrl, a.levelFromCtrl, Lk);
