@@ -21,7 +21,9 @@ carries:
   cc            connected components of the same resident graph;
   mbfs          batched multi-source BFS (128 roots per launch, 1,024 seeded roots) on
                 the same resident graph: aggregate GTEPS, the plain single-source GTEPS
-                of the same roots and the ratio of the two.
+                of the same roots (at the tuned single-source thresholds) and the ratio
+                of the two; the batches switch direction at their own thresholds
+                (--mbfs-alpha / --mbfs-beta).
 """
 from __future__ import annotations
 
