@@ -291,35 +291,42 @@ namespace {
 // and a pointer to the largest non-empty key.  Increments move the vertex at once;
 // decrements are only counted and applied when the vertex reaches the top (a vertex
 // whose stored key is the largest and has no pending decrements holds the true
-// maximum), so the window-eviction half of the updates costs one counter write.
+// maximum), so the window-eviction half of the updates costs one counter write.  A
+// vertex's key, pending count and list links share one 16-byte node (the updates are
+// random accesses over all n vertices: one cache line each instead of one per field),
+// and prefetch() lets a caller that knows its next targets start their loads early.
 class BucketQueue {
 public:
-    explicit BucketQueue(uint32_t n)
-        : key_(n, 0), pend_(n, 0), prev_(n), next_(n), in_(n, 1), head_(16, -1) {
+    explicit BucketQueue(uint32_t n) : nd_(n), head_(16, -1) {
         for (uint32_t v = 0; v < n; ++v) {
-            prev_[v] = int32_t(v) - 1;
-            next_[v] = (v + 1 < n) ? int32_t(v) + 1 : -1;
+            nd_[v].key = 0;
+            nd_[v].pend = 0;
+            nd_[v].prev = int32_t(v) - 1;
+            nd_[v].next = (v + 1 < n) ? int32_t(v) + 1 : -1;
         }
         if (n) head_[0] = 0;
         top_ = 0;
         size_ = n;
     }
-    bool contains(uint32_t v) const { return in_[v] != 0; }
+    bool contains(uint32_t v) const { return !(nd_[v].pend & kOut); }
     uint64_t size() const { return size_; }
+    void prefetch(uint32_t v) const { __builtin_prefetch(&nd_[v], 1, 1); }
     void remove(uint32_t v) {
         unlink(v);
-        in_[v] = 0;
+        nd_[v].pend |= kOut;
         --size_;
     }
     void inc(uint32_t v) {
-        if (!in_[v]) return;
+        Node& x = nd_[v];
+        if (x.pend & kOut) return;
         unlink(v);
-        ++key_[v];
+        ++x.key;
         link(v);
-        if (key_[v] > top_) top_ = key_[v];
+        if (x.key > top_) top_ = x.key;
     }
     void dec(uint32_t v) {
-        if (in_[v]) ++pend_[v];
+        Node& x = nd_[v];
+        if (!(x.pend & kOut)) ++x.pend;
     }
     // Largest effective key; among equal keys the most recently linked vertex.
     int64_t pop_max() {
@@ -327,10 +334,11 @@ public:
             while (top_ > 0 && head_[top_] < 0) --top_;
             const int32_t v = head_[top_];
             if (v < 0) return -1;
-            if (pend_[v]) {
+            Node& x = nd_[v];
+            if (x.pend) {
                 unlink(uint32_t(v));
-                key_[v] -= pend_[v];
-                pend_[v] = 0;
+                x.key -= x.pend;
+                x.pend = 0;
                 link(uint32_t(v));
                 continue;
             }
@@ -340,23 +348,27 @@ public:
     }
 
 private:
+    static constexpr uint32_t kOut = 0x80000000u;  // pend's top bit: not in the queue
+    struct Node {
+        uint32_t key, pend;
+        int32_t prev, next;
+    };
     void unlink(uint32_t v) {
-        const uint32_t k = key_[v];
-        if (prev_[v] >= 0) next_[prev_[v]] = next_[v];
-        else head_[k] = next_[v];
-        if (next_[v] >= 0) prev_[next_[v]] = prev_[v];
+        const Node& x = nd_[v];
+        if (x.prev >= 0) nd_[x.prev].next = x.next;
+        else head_[x.key] = x.next;
+        if (x.next >= 0) nd_[x.next].prev = x.prev;
     }
     void link(uint32_t v) {
-        const uint32_t k = key_[v];
+        Node& x = nd_[v];
+        const uint32_t k = x.key;
         if (k >= head_.size()) head_.resize(std::max<size_t>(size_t(k) + 1, head_.size() * 2), -1);
-        prev_[v] = -1;
-        next_[v] = head_[k];
-        if (head_[k] >= 0) prev_[head_[k]] = int32_t(v);
+        x.prev = -1;
+        x.next = head_[k];
+        if (head_[k] >= 0) nd_[head_[k]].prev = int32_t(v);
         head_[k] = int32_t(v);
     }
-    std::vector<uint32_t> key_, pend_;
-    std::vector<int32_t> prev_, next_;
-    std::vector<uint8_t> in_;
+    std::vector<Node> nd_;
     std::vector<int32_t> head_;
     uint32_t top_;
     uint64_t size_;
@@ -397,16 +409,30 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
             pq.remove(v);
         }
     // Score change of every vertex related to v when v enters (+1) or leaves (-1) the window.
+    // (the targets of a run of updates are known ahead: their nodes are prefetched
+    // kAhead updates before they are touched; the updates themselves keep their order,
+    // on which ties depend)
+    constexpr uint32_t kAhead = 8;
     auto touch = [&](uint32_t v, bool add) {
         auto upd = [&](uint32_t x) { add ? pq.inc(x) : pq.dec(x); };
-        if (outDeg(v) <= huge)
-            for (uint32_t p = g.colPtrs[v]; p < g.colPtrs[v + 1]; ++p) upd(g.rows[p]);
-        for (uint32_t p = in->colPtrs[v]; p < in->colPtrs[v + 1]; ++p) {
+        auto run = [&](uint32_t p0, uint32_t p1) {
+            for (uint32_t p = p0; p < p1; ++p) {
+                if (p + kAhead < p1) pq.prefetch(g.rows[p + kAhead]);
+                upd(g.rows[p]);
+            }
+        };
+        if (outDeg(v) <= huge) run(g.colPtrs[v], g.colPtrs[v + 1]);
+        const uint32_t i1 = in->colPtrs[v + 1];
+        for (uint32_t p = in->colPtrs[v]; p < i1; ++p) {
+            if (p + kAhead < i1) {
+                const uint32_t ua = in->rows[p + kAhead];
+                pq.prefetch(ua);
+                __builtin_prefetch(&g.colPtrs[ua], 0, 1);
+            }
             uint32_t u = in->rows[p];
             if (outDeg(u) > huge) continue;
             upd(u);
-            if (outDeg(u) > 1)
-                for (uint32_t q = g.colPtrs[u]; q < g.colPtrs[u + 1]; ++q) upd(g.rows[q]);
+            if (outDeg(u) > 1) run(g.colPtrs[u], g.colPtrs[u + 1]);
         }
     };
     if (pq.size() > 0) {
