@@ -374,6 +374,125 @@ private:
     uint64_t size_;
 };
 
+// The same queue cut into T partitions (blocks of 16 vertices), each with its own
+// bucket lists, so that T threads apply the updates of one touch to disjoint nodes at
+// once.  Each link carries a stamp from one global sequence (an update's position in the
+// sequential update stream, or the moment of a relink at the top), so the most recently
+// linked vertex among equal keys is the head of largest stamp over the partitions' top
+// buckets: the choice the single queue makes, hence the same order.
+class SplitQueue {
+public:
+    SplitQueue(uint32_t n, unsigned parts) : nd_(n), part_(parts), T_(parts) {
+        for (auto& p : part_) {
+            p.head.assign(16, -1);
+            p.top = 0;
+        }
+        // bucket 0 holds every vertex, the lowest ID most recent (as the single queue
+        // links them): stamp n - v
+        std::vector<int32_t> last(T_, -1);
+        for (uint32_t v = 0; v < n; ++v) {
+            Node& x = nd_[v];
+            x.key = 0;
+            x.pend = 0;
+            x.stamp = uint64_t(n) - v;
+            const unsigned p = part_of(v);
+            x.prev = last[p];
+            x.next = -1;
+            if (last[p] >= 0) nd_[last[p]].next = int32_t(v);
+            else part_[p].head[0] = int32_t(v);
+            last[p] = int32_t(v);
+        }
+        size_ = n;
+    }
+    // partitions own blocks of 16 consecutive vertices, so that a cache line of nodes is
+    // written by one thread only; T is a power of two
+    unsigned part_of(uint32_t v) const { return (v >> 4) & (T_ - 1); }
+    bool contains(uint32_t v) const { return !(nd_[v].pend & kOut); }
+    uint64_t size() const { return size_; }
+    void prefetch(uint32_t v) const { __builtin_prefetch(&nd_[v], 1, 1); }
+    void remove(uint32_t v) {
+        unlink(v);
+        nd_[v].pend |= kOut;
+        --size_;
+    }
+    // only the thread of v's partition calls these during a touch
+    void inc(uint32_t v, uint64_t stamp) {
+        Node& x = nd_[v];
+        if (x.pend & kOut) return;
+        unlink(v);
+        ++x.key;
+        link(v, stamp);
+        Part& p = part_[part_of(v)];
+        if (x.key > p.top) p.top = x.key;
+    }
+    void dec(uint32_t v) {
+        Node& x = nd_[v];
+        if (!(x.pend & kOut)) ++x.pend;
+    }
+    // Largest effective key; among equal keys the most recently linked vertex.  A relink
+    // takes its stamp from `stamp`, which it advances.
+    int64_t pop_max(uint64_t& stamp) {
+        for (;;) {
+            int32_t v = -1;
+            uint32_t key = 0;
+            for (auto& p : part_) {
+                while (p.top > 0 && p.head[p.top] < 0) --p.top;
+                const int32_t h = p.head[p.top];
+                if (h < 0) continue;
+                if (v < 0 || p.top > key || (p.top == key && nd_[h].stamp > nd_[v].stamp)) {
+                    v = h;
+                    key = p.top;
+                }
+            }
+            if (v < 0) return -1;
+            Node& x = nd_[v];
+            if (x.pend) {
+                unlink(uint32_t(v));
+                x.key -= x.pend;
+                x.pend = 0;
+                link(uint32_t(v), stamp++);
+                continue;
+            }
+            remove(uint32_t(v));
+            return v;
+        }
+    }
+
+private:
+    static constexpr uint32_t kOut = 0x80000000u;
+    struct Node {
+        uint32_t key, pend;
+        int32_t prev, next;
+        uint64_t stamp;
+    };
+    struct Part {
+        std::vector<int32_t> head;
+        uint32_t top;
+        char pad[64];  // partitions written by different threads keep to their own lines
+    };
+    void unlink(uint32_t v) {
+        const Node& x = nd_[v];
+        if (x.prev >= 0) nd_[x.prev].next = x.next;
+        else part_[part_of(v)].head[x.key] = x.next;
+        if (x.next >= 0) nd_[x.next].prev = x.prev;
+    }
+    void link(uint32_t v, uint64_t stamp) {
+        Node& x = nd_[v];
+        std::vector<int32_t>& head = part_[part_of(v)].head;
+        const uint32_t k = x.key;
+        if (k >= head.size()) head.resize(std::max<size_t>(size_t(k) + 1, head.size() * 2), -1);
+        x.prev = -1;
+        x.next = head[k];
+        x.stamp = stamp;
+        if (head[k] >= 0) nd_[head[k]].prev = int32_t(v);
+        head[k] = int32_t(v);
+    }
+    std::vector<Node> nd_;
+    std::vector<Part> part_;
+    unsigned T_;
+    uint64_t size_;
+};
+
 }  // namespace
 
 // Gorder greedy (Wei, Yu, Lu, Lin, "Speedup Graph Processing by Graph Ordering",
@@ -385,7 +504,7 @@ private:
 // returns a vertex of exactly maximal score (the reference's UnitHeap halves pending
 // decrements when it re-files the top).  Unlike the reference (CSC.cuh:936-943) the
 // order is returned in the caller's vertex IDs.
-std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
+std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threads) {
     const uint32_t n = g.n;
     std::vector<uint32_t> inv(n);
     if (n == 0) return inv;
@@ -400,25 +519,19 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
     auto outDeg = [&](uint32_t v) { return g.colPtrs[v + 1] - g.colPtrs[v]; };
     auto inDeg = [&](uint32_t v) { return in->colPtrs[v + 1] - in->colPtrs[v]; };
     const uint64_t huge = uint64_t(std::sqrt(double(n)));
-    BucketQueue pq(n);
-    std::vector<uint32_t> order, zero;
-    order.reserve(n);
-    for (uint32_t v = 0; v < n; ++v)
-        if (outDeg(v) + inDeg(v) == 0) {
-            zero.push_back(v);
-            pq.remove(v);
-        }
-    // Score change of every vertex related to v when v enters (+1) or leaves (-1) the window.
-    // (the targets of a run of updates are known ahead: their nodes are prefetched
-    // kAhead updates before they are touched; the updates themselves keep their order,
-    // on which ties depend)
+    // The score changes of a touch of v (v entering (+1) or leaving (-1) the window), as a
+    // stream of targets: v's out-neighbours, then for each in-neighbour u (not a hub) u
+    // itself and u's out-neighbours.  walk() calls f(x, i) for the i-th target x of the
+    // stream (i counts every target, so stream positions are the same whoever walks it)
+    // and returns the stream's length.  The targets of a run are known ahead: their
+    // nodes are prefetched kAhead updates before they are touched.
     constexpr uint32_t kAhead = 8;
-    auto touch = [&](uint32_t v, bool add) {
-        auto upd = [&](uint32_t x) { add ? pq.inc(x) : pq.dec(x); };
+    auto walk = [&](uint32_t v, auto&& f, auto&& pre) -> uint64_t {
+        uint64_t i = 0;
         auto run = [&](uint32_t p0, uint32_t p1) {
             for (uint32_t p = p0; p < p1; ++p) {
-                if (p + kAhead < p1) pq.prefetch(g.rows[p + kAhead]);
-                upd(g.rows[p]);
+                if (p + kAhead < p1) pre(g.rows[p + kAhead]);
+                f(g.rows[p], i++);
             }
         };
         if (outDeg(v) <= huge) run(g.colPtrs[v], g.colPtrs[v + 1]);
@@ -426,16 +539,24 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
         for (uint32_t p = in->colPtrs[v]; p < i1; ++p) {
             if (p + kAhead < i1) {
                 const uint32_t ua = in->rows[p + kAhead];
-                pq.prefetch(ua);
+                pre(ua);
                 __builtin_prefetch(&g.colPtrs[ua], 0, 1);
             }
             uint32_t u = in->rows[p];
             if (outDeg(u) > huge) continue;
-            upd(u);
+            f(u, i++);
             if (outDeg(u) > 1) run(g.colPtrs[u], g.colPtrs[u + 1]);
         }
+        return i;
     };
-    if (pq.size() > 0) {
+    std::vector<uint32_t> order, zero;
+    order.reserve(n);
+    auto first_vertex = [&](auto& pq) {
+        for (uint32_t v = 0; v < n; ++v)
+            if (outDeg(v) + inDeg(v) == 0) {
+                zero.push_back(v);
+                pq.remove(v);
+            }
         uint32_t first = 0;
         int64_t best = -1;
         for (uint32_t v = 0; v < n; ++v)
@@ -443,17 +564,125 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window) {
                 best = inDeg(v);
                 first = v;
             }
-        pq.remove(first);
-        order.push_back(first);
-        touch(first, true);
-        // v is chosen against the last `window` placed vertices; then the vertex that
-        // falls out of the window is retired and v's contributions are added.
-        while (pq.size() > 0) {
-            int64_t v = pq.pop_max();
-            order.push_back(uint32_t(v));
-            if (order.size() > window) touch(order[order.size() - 1 - window], false);
-            touch(uint32_t(v), true);
+        return first;
+    };
+    threads = std::max(1u, threads);
+    if (threads == 1 || n < (1u << 16)) {
+        BucketQueue pq(n);
+        auto touch = [&](uint32_t v, bool add) {
+            walk(
+                v, [&](uint32_t x, uint64_t) { add ? pq.inc(x) : pq.dec(x); },
+                [&](uint32_t x) { pq.prefetch(x); });
+        };
+        const uint32_t first = first_vertex(pq);
+        if (pq.size() > 0) {
+            pq.remove(first);
+            order.push_back(first);
+            touch(first, true);
+            // v is chosen against the last `window` placed vertices; then the vertex that
+            // falls out of the window is retired and v's contributions are added.
+            while (pq.size() > 0) {
+                int64_t v = pq.pop_max();
+                order.push_back(uint32_t(v));
+                if (order.size() > window) touch(order[order.size() - 1 - window], false);
+                touch(uint32_t(v), true);
+            }
         }
+    } else {
+        // The same greedy with the queue split over the threads: a touch with many
+        // in-neighbours is walked once and its updates dealt to the partitions, each
+        // thread applying its own in stream order; a small touch is applied by the calling
+        // thread alone.  The random node updates are the cost (~3.5e10 at R-MAT scale
+        // 22), and they spread over the threads' memory accesses.
+        // Threads: a power of two, at most kMaxSplit (the workers wait for each touch by
+        // spinning, which pays only while every one of them has a core).  A touch's stream
+        // positions fit 32 bits: a stream is at most the graph's edges plus 2n.
+        constexpr unsigned kMaxSplit = 8;
+        threads = std::min(threads, kMaxSplit);
+        while (threads & (threads - 1)) threads &= threads - 1;
+        SplitQueue pq(n, threads);
+        uint64_t stamp = uint64_t(n) + 1;
+        constexpr uint32_t kSplitIn = 64;  // in-neighbours from which a touch is shared
+        // a shared touch: the calling thread walks the stream once and deals each update
+        // (target, stream position) to its partition's buffer; every thread then applies
+        // its buffer in order
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> buf(threads);
+        std::atomic<uint64_t> epoch{0};
+        std::atomic<unsigned> doneCount{0};
+        std::atomic<bool> stop{false};
+        bool jobAdd = false;
+        uint64_t jobBase = 0;
+        auto apply = [&](unsigned t) {
+            const auto& b = buf[t];
+            const size_t m = b.size();
+            for (size_t k = 0; k < m; ++k) {
+                if (k + 8 < m) pq.prefetch(b[k + 8].first);
+                jobAdd ? pq.inc(b[k].first, jobBase + b[k].second) : pq.dec(b[k].first);
+            }
+        };
+        auto spin_wait = [](auto&& ready) {
+            for (unsigned spins = 0; !ready();)
+                if (++spins >= 4096) {
+                    spins = 0;
+                    std::this_thread::yield();
+                }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < threads; ++t)
+            pool.emplace_back([&, t] {
+                uint64_t seen = 0;
+                for (;;) {
+                    uint64_t e = seen;
+                    spin_wait([&] {
+                        e = epoch.load(std::memory_order_acquire);
+                        return e != seen || stop.load(std::memory_order_relaxed);
+                    });
+                    if (e == seen) return;  // stopped
+                    seen = e;
+                    apply(t);
+                    doneCount.fetch_add(1, std::memory_order_release);
+                }
+            });
+        auto touch = [&](uint32_t v, bool add) {
+            if (inDeg(v) < kSplitIn) {
+                stamp += walk(
+                    v,
+                    [&](uint32_t x, uint64_t i) { add ? pq.inc(x, stamp + i) : pq.dec(x); },
+                    [&](uint32_t x) { pq.prefetch(x); });
+                return;
+            }
+            for (auto& b : buf) b.clear();
+            const uint64_t len = walk(
+                v, [&](uint32_t x, uint64_t i) { buf[pq.part_of(x)].emplace_back(x, uint32_t(i)); },
+                [](uint32_t) {});
+            jobAdd = add;
+            jobBase = stamp;
+            doneCount.store(0, std::memory_order_relaxed);
+            epoch.fetch_add(1, std::memory_order_release);
+            apply(0);
+            spin_wait([&] { return doneCount.load(std::memory_order_acquire) == threads - 1; });
+            stamp += len;
+        };
+        try {
+            const uint32_t first = first_vertex(pq);
+            if (pq.size() > 0) {
+                pq.remove(first);
+                order.push_back(first);
+                touch(first, true);
+                while (pq.size() > 0) {
+                    int64_t v = pq.pop_max(stamp);
+                    order.push_back(uint32_t(v));
+                    if (order.size() > window) touch(order[order.size() - 1 - window], false);
+                    touch(uint32_t(v), true);
+                }
+            }
+        } catch (...) {
+            stop.store(true);
+            for (auto& th : pool) th.join();
+            throw;
+        }
+        stop.store(true);
+        for (auto& th : pool) th.join();
     }
     order.insert(order.end(), zero.begin(), zero.end());
     for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
@@ -485,9 +714,9 @@ std::vector<uint32_t> reorder(CSC& g, const blest_reorder_opts& o) {
         case BLEST_REORDER_DEGREE: inv = order_degree(g, true); break;
         case BLEST_REORDER_RCM: inv = order_rcm(g, o.seed); break;
         case BLEST_REORDER_JACCARD: inv = order_jaccard_window(g, slice, o.window, threads); break;
-        case BLEST_REORDER_GORDER: inv = order_gorder(g, o.window); break;
+        case BLEST_REORDER_GORDER: inv = order_gorder(g, o.window, threads); break;
         case BLEST_REORDER_GORDER_JACCARD: {  // CSC::gorderWithJackard (CSC.cuh:932-948)
-            std::vector<uint32_t> first = order_gorder(g, o.window);
+            std::vector<uint32_t> first = order_gorder(g, o.window, threads);
             csc_apply_permutation(g, first.data(), threads);
             std::vector<uint32_t> second = order_jaccard_window(g, slice, o.window, threads);
             csc_apply_permutation(g, second.data(), threads);

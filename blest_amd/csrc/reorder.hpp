@@ -16,7 +16,9 @@ std::vector<uint32_t> order_degree(const CSC& g, bool ascending);
 std::vector<uint32_t> order_rcm(const CSC& g, uint64_t seed);
 std::vector<uint32_t> order_jaccard_window(const CSC& g, uint32_t sliceSize, uint32_t window,
                                            unsigned threads);
-std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window);
+// threads > 1: the greedy's priority updates spread over up to 8 threads (the same order
+// as with one thread).
+std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threads = 1);
 
 // chainPermutations (Common.cuh:81-91): result[i] = second[first[i]].
 std::vector<uint32_t> chain_permutations(const std::vector<uint32_t>& first,
