@@ -526,7 +526,9 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
     // and returns the stream's length.  The targets of a run are known ahead: their
     // nodes are prefetched kAhead updates before they are touched.
     constexpr uint32_t kAhead = 8;
-    auto walk = [&](uint32_t v, auto&& f, auto&& pre) -> uint64_t {
+    // walk_part: the part of the stream from in-neighbour entries [pA, pB) of v, after
+    // v's own out-list when withOut (the stream is the concatenation of such parts).
+    auto walk_part = [&](uint32_t v, bool withOut, uint32_t pA, uint32_t i1, auto&& f, auto&& pre) -> uint64_t {
         uint64_t i = 0;
         auto run = [&](uint32_t p0, uint32_t p1) {
             for (uint32_t p = p0; p < p1; ++p) {
@@ -534,9 +536,8 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
                 f(g.rows[p], i++);
             }
         };
-        if (outDeg(v) <= huge) run(g.colPtrs[v], g.colPtrs[v + 1]);
-        const uint32_t i1 = in->colPtrs[v + 1];
-        for (uint32_t p = in->colPtrs[v]; p < i1; ++p) {
+        if (withOut && outDeg(v) <= huge) run(g.colPtrs[v], g.colPtrs[v + 1]);
+        for (uint32_t p = pA; p < i1; ++p) {
             if (p + kAhead < i1) {
                 const uint32_t ua = in->rows[p + kAhead];
                 pre(ua);
@@ -548,6 +549,9 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
             if (outDeg(u) > 1) run(g.colPtrs[u], g.colPtrs[u + 1]);
         }
         return i;
+    };
+    auto walk = [&](uint32_t v, auto&& f, auto&& pre) -> uint64_t {
+        return walk_part(v, true, in->colPtrs[v], in->colPtrs[v + 1], f, pre);
     };
     std::vector<uint32_t> order, zero;
     order.reserve(n);
@@ -590,7 +594,7 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
         }
     } else {
         // The same greedy with the queue split over the threads: a touch with many
-        // in-neighbours is walked once and its updates dealt to the partitions, each
+        // in-neighbours is walked in slices and its updates dealt to the partitions, each
         // thread applying its own in stream order; a small touch is applied by the calling
         // thread alone.  The random node updates are the cost (~3.5e10 at R-MAT scale
         // 22), and they spread over the threads' memory accesses.
@@ -603,23 +607,21 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
         SplitQueue pq(n, threads);
         uint64_t stamp = uint64_t(n) + 1;
         constexpr uint32_t kSplitIn = 64;  // in-neighbours from which a touch is shared
-        // a shared touch: the calling thread walks the stream once and deals each update
-        // (target, stream position) to its partition's buffer; every thread then applies
-        // its buffer in order
-        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> buf(threads);
+        // a shared touch runs in two phases, each thread taking one share of each:
+        //   walk   thread w walks a contiguous slice of v's in-neighbours (thread 0 first
+        //          v's own out-list) and deals each update, with its position in the
+        //          slice, to buffer [w][partition of the target];
+        //   apply  thread t applies buffers [0][t], [1][t], ... in that order, the
+        //          positions offset by the lengths of the slices before, i.e. every update
+        //          of its partition in stream order with its stream position as stamp.
+        std::vector<std::vector<std::vector<std::pair<uint32_t, uint32_t>>>> buf(
+            threads, std::vector<std::vector<std::pair<uint32_t, uint32_t>>>(threads));
+        std::vector<uint64_t> sliceLen(threads);
         std::atomic<uint64_t> epoch{0};
-        std::atomic<unsigned> doneCount{0};
         std::atomic<bool> stop{false};
+        uint32_t jobV = 0;
         bool jobAdd = false;
         uint64_t jobBase = 0;
-        auto apply = [&](unsigned t) {
-            const auto& b = buf[t];
-            const size_t m = b.size();
-            for (size_t k = 0; k < m; ++k) {
-                if (k + 8 < m) pq.prefetch(b[k + 8].first);
-                jobAdd ? pq.inc(b[k].first, jobBase + b[k].second) : pq.dec(b[k].first);
-            }
-        };
         auto spin_wait = [](auto&& ready) {
             for (unsigned spins = 0; !ready();)
                 if (++spins >= 4096) {
@@ -627,10 +629,41 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
                     std::this_thread::yield();
                 }
         };
+        // all threads (the caller included) meet here; `met` counts the thread's own
+        // meetings so far (every thread has as many), so the counter never needs a reset
+        std::atomic<uint64_t> arrivedAll{0};
+        auto meet = [&](uint64_t& met) {
+            ++met;
+            arrivedAll.fetch_add(1, std::memory_order_acq_rel);
+            const uint64_t target = uint64_t(threads) * met;
+            spin_wait([&] { return arrivedAll.load(std::memory_order_acquire) >= target; });
+        };
+        auto walk_share = [&](unsigned w) {
+            const uint32_t a0 = in->colPtrs[jobV], a1 = in->colPtrs[jobV + 1];
+            const uint32_t pA = a0 + uint32_t(uint64_t(a1 - a0) * w / threads);
+            const uint32_t pB = a0 + uint32_t(uint64_t(a1 - a0) * (w + 1) / threads);
+            auto& mine = buf[w];
+            for (auto& b : mine) b.clear();
+            sliceLen[w] = walk_part(
+                jobV, w == 0, pA, pB, [&](uint32_t x, uint64_t i) { mine[pq.part_of(x)].emplace_back(x, uint32_t(i)); },
+                [](uint32_t) {});
+        };
+        auto apply_share = [&](unsigned t) {
+            uint64_t off = jobBase;
+            for (unsigned w = 0; w < threads; ++w) {
+                const auto& b = buf[w][t];
+                const size_t m = b.size();
+                for (size_t k = 0; k < m; ++k) {
+                    if (k + 8 < m) pq.prefetch(b[k + 8].first);
+                    jobAdd ? pq.inc(b[k].first, off + b[k].second) : pq.dec(b[k].first);
+                }
+                off += sliceLen[w];
+            }
+        };
         std::vector<std::thread> pool;
         for (unsigned t = 1; t < threads; ++t)
             pool.emplace_back([&, t] {
-                uint64_t seen = 0;
+                uint64_t seen = 0, met = 0;
                 for (;;) {
                     uint64_t e = seen;
                     spin_wait([&] {
@@ -639,10 +672,13 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
                     });
                     if (e == seen) return;  // stopped
                     seen = e;
-                    apply(t);
-                    doneCount.fetch_add(1, std::memory_order_release);
+                    walk_share(t);
+                    meet(met);
+                    apply_share(t);
+                    meet(met);
                 }
             });
+        uint64_t callerMet = 0;
         auto touch = [&](uint32_t v, bool add) {
             if (inDeg(v) < kSplitIn) {
                 stamp += walk(
@@ -651,17 +687,15 @@ std::vector<uint32_t> order_gorder(const CSC& g, uint32_t window, unsigned threa
                     [&](uint32_t x) { pq.prefetch(x); });
                 return;
             }
-            for (auto& b : buf) b.clear();
-            const uint64_t len = walk(
-                v, [&](uint32_t x, uint64_t i) { buf[pq.part_of(x)].emplace_back(x, uint32_t(i)); },
-                [](uint32_t) {});
+            jobV = v;
             jobAdd = add;
             jobBase = stamp;
-            doneCount.store(0, std::memory_order_relaxed);
             epoch.fetch_add(1, std::memory_order_release);
-            apply(0);
-            spin_wait([&] { return doneCount.load(std::memory_order_acquire) == threads - 1; });
-            stamp += len;
+            walk_share(0);
+            meet(callerMet);
+            apply_share(0);
+            meet(callerMet);
+            for (unsigned w = 0; w < threads; ++w) stamp += sliceLen[w];
         };
         try {
             const uint32_t first = first_vertex(pq);
