@@ -118,7 +118,7 @@ typedef struct blest_reorder_opts {
     uint32_t window;     /* Jaccard / Gorder window (reference -w); multiple of slice_size */
     int jaccard;         /* reference -j, used by AUTO */
     int gorder;          /* AUTO: route social&jaccard to GORDER_JACCARD */
-    uint32_t threads;    /* 0 = all hardware threads */
+    uint32_t threads;    /* 0 = all hardware threads (Gorder: at most 8); every count gives the same order */
     uint64_t seed;       /* RCM pseudo-peripheral restarts, RANDOM */
 } blest_reorder_opts;
 void blest_reorder_opts_default(blest_reorder_opts* o);
